@@ -1,0 +1,125 @@
+/*
+ * mdft — MI355X-native 2-D DFT (drop-in for the 2-D DFT path of cppd/math).
+ *
+ * C ABI: plain pointers and sizes.  Data is row-major complex<float>
+ * interleaved (re, im), element (x, y) at index y*width + x, width the fast
+ * axis (src/gpu/dft/code/dft_mul.comp:94, src/dft/fftw.cpp:95).  Forward is
+ * unnormalised, X[k] = sum x[j] exp(-2*pi*i*(j1*k1/w + j2*k2/h)); inverse uses
+ * exp(+...) and is scaled by 1/(w*h) (src/gpu/dft/dft.cpp:326-347 with the
+ * diagonals of dft.cpp:77-111; src/dft/fftw.cpp:140-150).  A size of 1 along
+ * an axis leaves that axis unchanged (dft.cpp:330,339).
+ *
+ * Every entry point returns MDFT_OK (0) or a negative MDFT_ERR_* code; the
+ * message of the last error on the calling thread is mdft_last_error().  The
+ * message texts of the reference are kept where the reference has one.
+ *
+ * Threading: a plan or batch is used by one thread at a time (the reference's
+ * Dft asserts thread affinity, dft.cpp:140,264,313,328); distinct plans may
+ * be used concurrently.  The server's submit/poll/wait are thread-safe.
+ */
+#ifndef MDFT_H
+#define MDFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDFT_OK 0
+#define MDFT_ERR_INVALID (-1)     /* null pointer or bad argument */
+#define MDFT_ERR_SIZE (-2)        /* "Error size <n> for compute m" (function.cpp:34-38) */
+#define MDFT_ERR_NO_BUFFERS (-3)  /* "No DFT buffers" (compute.cpp:259-262) */
+#define MDFT_ERR_DATA_SIZE (-4)   /* "Wrong DFT buffer size" (compute.cpp:264-267) */
+#define MDFT_ERR_DEVICE (-5)      /* HIP runtime failure */
+#define MDFT_ERR_BUDGET (-6)      /* request larger than the server's HBM budget */
+#define MDFT_ERR_TIMEOUT (-7)     /* a bounded device-side wait expired */
+#define MDFT_ERR_STATE (-8)       /* call not valid in the object's current state */
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Message of the last error raised on this thread ("" if none). */
+const char* mdft_last_error(void);
+
+/* Library version string. */
+const char* mdft_version(void);
+
+/* Select the HIP device used by objects created afterwards on this thread
+ * and build its twiddle tables.  Optional: device 0 is initialised lazily.
+ * Replaces the reference's device choice of
+ * create_compute_vector(DeviceSearchType) (src/gpu/dft/compute.h:68,
+ * compute.cpp:286-305). */
+int mdft_init(int device);
+
+/* ---- one transform ------------------------------------------------------- */
+
+typedef struct mdft_plan_s* mdft_plan;
+
+/* Plan a width x height transform: padded sizes, FP64-generated chirp and
+ * diagonal tables resident on the device.  Replaces
+ * ComputeVector::create_buffers(width, height) (src/gpu/dft/compute.h:56,
+ * compute.cpp:229-255) and Dft::create_buffers (dft.h:38, dft.cpp:262-309).
+ * Errors: MDFT_ERR_SIZE with "Error size <n> for compute m" when a side is
+ * outside [1, 2^29]. */
+int mdft_plan_create(mdft_plan* plan, uint32_t width, uint32_t height);
+
+/* Free a plan and its device memory (ComputeVector destructor /
+ * Dft::delete_buffers, dft.h:39, dft.cpp:311-324).  NULL is accepted. */
+int mdft_plan_destroy(mdft_plan plan);
+
+/* Padded lengths the plan uses per axis (compute_m, function.cpp:32-45; 1
+ * when the axis is a power of two and needs no padding) and the plan's
+ * device bytes. */
+int mdft_plan_info(mdft_plan plan, uint32_t* m1, uint32_t* m2, uint64_t* device_bytes);
+
+/* In-place transform of count complex<float> values in host memory, blocking
+ * until the result is back.  Replaces ComputeVector::exec(inverse, &vector)
+ * (src/gpu/dft/compute.h:58, compute.cpp:257-283) and ns::dft::DFT::exec
+ * (src/dft/dft.h:29).  Errors: MDFT_ERR_NO_BUFFERS "No DFT buffers" for a
+ * NULL plan; MDFT_ERR_DATA_SIZE "Wrong DFT buffer size" when data is NULL or
+ * count != width*height. */
+int mdft_exec(mdft_plan plan, int inverse, void* data, size_t count);
+
+/* Enqueue the transform of a device image on a HIP stream (hipStream_t, NULL
+ * for the default stream) without synchronising; src may equal dst.  The
+ * analogue of recording Dft::compute_commands(command_buffer, inverse) into a
+ * caller's command buffer (src/gpu/dft/dft.h:41, dft.cpp:326-347). */
+int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, void* stream);
+
+/* ---- many transforms of mixed sizes in one persistent launch ------------- */
+
+typedef struct {
+    uint32_t width;
+    uint32_t height;
+    int32_t inverse;
+    uint32_t reserved;
+    uint64_t src_offset; /* element (complex<float>) offsets into the base pointers */
+    uint64_t dst_offset;
+} mdft_desc;
+
+typedef struct mdft_batch_s* mdft_batch;
+
+#define MDFT_BATCH_GRAPH 1u       /* replay the launch as a captured hipGraph */
+#define MDFT_BATCH_TWO_PHASE 2u   /* rows and columns as two launches instead of dependency counters */
+
+/* Plan a batch of transforms (no counterpart in the reference, whose exec
+ * handles one image per call).  Every side must have a padded length
+ * compute_m(side) <= 8192: powers of two up to 8192, other sides up to 4097. */
+int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count, uint32_t flags);
+
+/* Enqueue the whole batch on a stream: one persistent kernel (plus a counter
+ * reset) for all transforms.  src may equal dst (in place per transform). */
+int mdft_batch_exec_device(mdft_batch batch, const void* src, void* dst, void* stream);
+
+/* Device error flag of the last completed launch: 0, or MDFT_ERR_TIMEOUT if
+ * a dependency wait expired.  Synchronises the batch's last stream. */
+int mdft_batch_check(mdft_batch batch);
+
+int mdft_batch_destroy(mdft_batch batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDFT_H */

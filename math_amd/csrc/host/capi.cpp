@@ -1,0 +1,120 @@
+// C ABI of the single-transform path: plans, host and device exec, errors.
+#include "capi_util.hpp"
+
+#include <cstring>
+#include <string>
+
+using mdft::Context;
+using mdft::Error;
+using mdft::LinePlan;
+
+struct mdft_plan_s {
+    int device = 0;
+    uint32_t w = 0, h = 0;
+    std::shared_ptr<const LinePlan> row, col;
+    float2* xd = nullptr;          // device image for mdft_exec, allocated on first use
+    hipStream_t stream = nullptr;  // private stream of mdft_exec
+};
+
+namespace mdft {
+namespace {
+thread_local std::string t_error;
+}
+void set_error(const std::string& s) { t_error = s; }
+const char* get_error() { return t_error.c_str(); }
+}  // namespace mdft
+
+extern "C" {
+
+const char* mdft_last_error(void) { return mdft::get_error(); }
+
+const char* mdft_version(void) { return "mdft 0.1 (gfx950)"; }
+
+int mdft_init(int device)
+{
+    MDFT_API_BEGIN
+    Context::get(device);
+    Context::set_thread_device(device);
+    MDFT_API_END
+}
+
+int mdft_plan_create(mdft_plan* plan, uint32_t width, uint32_t height)
+{
+    MDFT_API_BEGIN
+    if (!plan) throw Error(MDFT_ERR_INVALID, "mdft_plan_create: plan is NULL");
+    *plan = nullptr;
+    // validate both sides with the reference's message before touching the device
+    mdft::compute_m((long long)width);
+    mdft::compute_m((long long)height);
+    Context& ctx = Context::get(Context::current_device());
+    auto p = std::make_unique<mdft_plan_s>();
+    p->device = ctx.device();
+    p->w = width;
+    p->h = height;
+    p->row = ctx.line_plan((int)width);
+    p->col = ctx.line_plan((int)height);
+    *plan = p.release();
+    MDFT_API_END
+}
+
+int mdft_plan_destroy(mdft_plan plan)
+{
+    MDFT_API_BEGIN
+    if (!plan) return MDFT_OK;
+    {
+        mdft::DeviceGuard g(plan->device);
+        if (plan->stream) {
+            (void)hipStreamSynchronize(plan->stream);
+            (void)hipStreamDestroy(plan->stream);
+        }
+        if (plan->xd) (void)hipFree(plan->xd);
+    }
+    delete plan;
+    MDFT_API_END
+}
+
+int mdft_plan_info(mdft_plan plan, uint32_t* m1, uint32_t* m2, uint64_t* device_bytes)
+{
+    MDFT_API_BEGIN
+    if (!plan) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
+    if (m1) *m1 = (uint32_t)plan->row->m;
+    if (m2) *m2 = (uint32_t)plan->col->m;
+    if (device_bytes)
+        *device_bytes = plan->row->bytes + (plan->col != plan->row ? plan->col->bytes : 0) +
+                        (plan->xd ? sizeof(float2) * (uint64_t)plan->w * plan->h : 0);
+    MDFT_API_END
+}
+
+int mdft_exec(mdft_plan plan, int inverse, void* data, size_t count)
+{
+    MDFT_API_BEGIN
+    // src/gpu/dft/compute.cpp:259-267
+    if (!plan || plan->w == 0 || plan->h == 0) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
+    if (!data || count != (size_t)plan->w * plan->h) throw Error(MDFT_ERR_DATA_SIZE, "Wrong DFT buffer size");
+    mdft::DeviceGuard g(plan->device);
+    const size_t bytes = sizeof(float2) * count;
+    if (!plan->stream) mdft::hip_check(hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (!plan->xd) mdft::hip_check(hipMalloc(&plan->xd, bytes), "hipMalloc image");
+    Context& ctx = Context::get(plan->device);
+    mdft::hip_check(hipMemcpyAsync(plan->xd, data, bytes, hipMemcpyHostToDevice, plan->stream), "copy in");
+    ctx.enqueue_transform(*plan->row, *plan->col, (int)plan->w, (int)plan->h, inverse != 0, plan->xd, plan->xd,
+                          plan->stream);
+    mdft::hip_check(hipMemcpyAsync(data, plan->xd, bytes, hipMemcpyDeviceToHost, plan->stream), "copy out");
+    mdft::hip_check(hipStreamSynchronize(plan->stream), "exec");
+    MDFT_API_END
+}
+
+int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, void* stream)
+{
+    MDFT_API_BEGIN
+    if (!plan || plan->w == 0 || plan->h == 0) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
+    if (!src || !dst) throw Error(MDFT_ERR_DATA_SIZE, "Wrong DFT buffer size");
+    mdft::DeviceGuard g(plan->device);
+    Context& ctx = Context::get(plan->device);
+    ctx.enqueue_transform(*plan->row, *plan->col, (int)plan->w, (int)plan->h, inverse != 0,
+                          static_cast<const float2*>(src), static_cast<float2*>(dst),
+                          static_cast<hipStream_t>(stream));
+    MDFT_API_END
+}
+
+}  // extern "C"

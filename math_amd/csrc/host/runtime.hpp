@@ -1,0 +1,95 @@
+// Host runtime of the 2-D DFT: per-device context (twiddle table, cache of
+// per-length line plans), transform enqueueing and error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+
+#include "../kernels/launch.hpp"
+
+namespace mdft {
+
+// Exception carrying an MDFT_ERR_* code; the C ABI turns it into a return
+// code plus a thread-local message.
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& msg) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// compute_m of src/gpu/dft/function.cpp:32-45; throws "Error size <n> for
+// compute m" outside [1, 2^29].
+int compute_m(long long n);
+int ilog2(uint64_t v);
+bool is_pow2(uint64_t v);
+
+// Device tables of one line length n.
+struct LinePlan {
+    int n = 0;
+    int m = 1;         // FFT length: n for a power of two, compute_m(n) otherwise
+    int logm = 0;
+    bool blue = false;  // Bluestein (n not a power of two)
+    float2* chirp = nullptr;  // n entries, Bluestein only
+    float2* dtab = nullptr;   // m entries, Bluestein only
+    uint64_t bytes = 0;
+    ~LinePlan();
+};
+
+class Context {
+public:
+    static Context& get(int device);
+    static int current_device();
+    static void set_thread_device(int device);
+
+    int device() const { return device_; }
+    int cu_count() const { return cu_count_; }
+    const float2* twiddles(int logm) const { return twiddles_ + tw_off(logm); }
+
+    // Cached, device-resident plan of one line length (thread-safe).
+    std::shared_ptr<const LinePlan> line_plan(int n);
+    uint64_t cached_bytes();
+
+    // Enqueue one axis pass of one image.
+    void enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
+                      bool inverse, bool cols, hipStream_t stream) const;
+    // Enqueue a whole w x h transform (rows then columns).
+    void enqueue_transform(const LinePlan& row, const LinePlan& col, int w, int h, bool inverse, const float2* src,
+                           float2* dst, hipStream_t stream) const;
+
+    AxisJob make_job(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
+                     bool inverse) const;
+
+private:
+    explicit Context(int device);
+    int device_;
+    int cu_count_ = 0;
+    float2* twiddles_ = nullptr;
+    hipStream_t gen_stream_ = nullptr;
+    double* scratch64_ = nullptr;
+    size_t scratch64_doubles_ = 0;
+    std::mutex mu_;
+    std::unordered_map<int, std::shared_ptr<const LinePlan>> lines_;
+};
+
+// RAII guard that makes a device current for the calling thread.
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int device);
+    ~DeviceGuard();
+
+private:
+    int prev_;
+};
+
+}  // namespace mdft

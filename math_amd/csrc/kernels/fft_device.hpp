@@ -1,0 +1,394 @@
+// Device-side building blocks of the 2-D DFT line passes (gfx950 / CDNA4).
+//
+// One "axis pass" transforms `count` lines of logical length n, laid out with
+// element stride es and line stride ls inside a row-major complex<float>
+// image (rows: es = 1, ls = w; columns: es = w, ls = 1).  A line of length n
+// is transformed through an FFT of length M = 2^LOGM: directly when n is a
+// power of two (M = n), otherwise by Bluestein's chirp-z method with M the
+// reference's padded length (src/gpu/dft/function.cpp:32-45) and the chirp
+// tables of function.cpp:49-92 generated in FP64 on the device.
+//
+// A workgroup owns LPW whole lines (a "tile").  Each thread holds EPT = 16
+// points of one line in VGPRs, at positions t + i*TPL (TPL = M/16 threads per
+// line).  The FFT is a radix-16 Stockham autosort (passes of 16 and one
+// remainder pass), DIT form: pass p gathers its operands from LDS at the
+// thread's fixed positions, multiplies by the per-pass twiddle row, runs a
+// radix-R butterfly in registers and scatters to the Stockham positions.  The
+// first pass needs no twiddles and reads straight from HBM; the last pass
+// leaves its results at the thread's own positions, so it writes straight to
+// HBM and Bluestein's two FFTs chain without an LDS exchange.  LDS holds one
+// exchange buffer, split into real and imaginary planes with one pad word per
+// 16 to keep the scatter conflict-free.
+//
+// Only the forward FFT exists: an inverse transform is conj(FFT(conj(x))),
+// folded into the load, the D multiply and the store.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdft {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// ---------------------------------------------------------------------------
+// compile-time line geometry
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int ept_of(int logm) { return logm < 4 ? (1 << logm) : 16; }
+__host__ __device__ constexpr int npass_of(int logm) { return (logm + 3) / 4; }
+__host__ __device__ constexpr int pass_bits(int logm, int p) { return (logm - 4 * p) >= 4 ? 4 : (logm - 4 * p); }
+
+// Twiddle rows of pass p >= 1 (Ns = 16^p before the pass, radix R):
+// row k (k < Ns) holds R entries exp(-2*pi*i*k*r/(Ns*R)), r < R.
+__host__ __device__ constexpr int64_t tw_pass_len(int logm, int p)
+{
+    return (int64_t(1) << (4 * p)) * (int64_t(1) << pass_bits(logm, p));
+}
+__host__ __device__ constexpr int64_t tw_pass_off(int logm, int p)
+{
+    int64_t off = 0;
+    for (int q = 1; q < p; ++q) off += tw_pass_len(logm, q);
+    return off;
+}
+__host__ __device__ constexpr int64_t tw_len(int logm)
+{
+    int64_t s = 0;
+    for (int p = 1; p < npass_of(logm); ++p) s += tw_pass_len(logm, p);
+    return s;
+}
+// All LOGM in [1, MAX_LOGM] share one table; offset of LOGM's block.
+constexpr int MAX_LOGM = 14;
+__host__ __device__ constexpr int64_t tw_off(int logm)
+{
+    int64_t s = 0;
+    for (int l = 1; l < logm; ++l) s += tw_len(l);
+    return s;
+}
+__host__ __device__ constexpr int64_t tw_total() { return tw_off(MAX_LOGM + 1); }
+
+__host__ __device__ constexpr int pad_pos(int p) { return p + (p >> 4); }
+
+template <int LOGM, int THREADS>
+struct LineCfg {
+    static constexpr int M = 1 << LOGM;
+    static constexpr int EPT = ept_of(LOGM);
+    static constexpr int TPL = M / EPT;          // threads per line
+    static constexpr int LPW = THREADS / TPL;    // lines per tile
+    static constexpr int NPASS = npass_of(LOGM);
+    static_assert(TPL <= THREADS, "line does not fit the workgroup");
+    static_assert(LPW * TPL == THREADS, "workgroup must hold whole lines");
+    static constexpr int PADM = M + (M >> 4);
+    // line stride in LDS words, offset so the lines of one wave land on
+    // distinct banks for both thread mappings
+    static constexpr int want_rows = TPL >= 32 ? 0 : TPL;
+    static constexpr int want_cols = LPW >= 32 ? 1 : 32 / LPW;
+    template <bool COLS>
+    static constexpr int lstride()
+    {
+        const int want = COLS ? want_cols : want_rows;
+        int s = PADM;
+        while ((s & 31) != (want & 31)) ++s;
+        return s;
+    }
+    template <bool COLS>
+    static constexpr int lds_floats()
+    {
+        return NPASS > 1 ? 2 * LPW * lstride<COLS>() : 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// radix-R forward DFT in registers (R <= 16), natural order in and out
+// ---------------------------------------------------------------------------
+// cos(2*pi*m/16), sin(2*pi*m/16)
+__device__ constexpr float kC16[16] = {1.0f,
+                                       0.92387953251128675613f,
+                                       0.70710678118654752440f,
+                                       0.38268343236508977173f,
+                                       0.0f,
+                                       -0.38268343236508977173f,
+                                       -0.70710678118654752440f,
+                                       -0.92387953251128675613f,
+                                       -1.0f,
+                                       -0.92387953251128675613f,
+                                       -0.70710678118654752440f,
+                                       -0.38268343236508977173f,
+                                       0.0f,
+                                       0.38268343236508977173f,
+                                       0.70710678118654752440f,
+                                       0.92387953251128675613f};
+__device__ constexpr float kS16[16] = {0.0f,
+                                       0.38268343236508977173f,
+                                       0.70710678118654752440f,
+                                       0.92387953251128675613f,
+                                       1.0f,
+                                       0.92387953251128675613f,
+                                       0.70710678118654752440f,
+                                       0.38268343236508977173f,
+                                       0.0f,
+                                       -0.38268343236508977173f,
+                                       -0.70710678118654752440f,
+                                       -0.92387953251128675613f,
+                                       -1.0f,
+                                       -0.92387953251128675613f,
+                                       -0.70710678118654752440f,
+                                       -0.38268343236508977173f};
+
+// d * exp(-2*pi*i*m/16) for a compile-time m
+template <int m>
+__device__ __forceinline__ float2 twc(float2 d)
+{
+    if constexpr (m == 0) {
+        return d;
+    } else if constexpr (m == 4) {
+        return make_float2(d.y, -d.x);
+    } else if constexpr (m == 2) {
+        const float r = 0.70710678118654752440f;
+        return make_float2(r * (d.x + d.y), r * (d.y - d.x));
+    } else if constexpr (m == 6) {
+        const float r = 0.70710678118654752440f;
+        return make_float2(r * (d.y - d.x), -r * (d.x + d.y));
+    } else {
+        const float c = kC16[m], s = kS16[m];
+        return make_float2(d.x * c + d.y * s, d.y * c - d.x * s);
+    }
+}
+
+__host__ __device__ constexpr int bitrev_small(int v, int bits)
+{
+    int r = 0;
+    for (int i = 0; i < bits; ++i) {
+        r = (r << 1) | (v & 1);
+        v >>= 1;
+    }
+    return r;
+}
+
+// one radix-2 DIF stage of span HALF over R points
+template <int R, int HALF, int BASE, int K>
+__device__ __forceinline__ void dif_bfly(float2* u)
+{
+    if constexpr (BASE < R) {
+        if constexpr (K < HALF) {
+            const float2 a = u[BASE + K], b = u[BASE + K + HALF];
+            u[BASE + K] = cadd(a, b);
+            u[BASE + K + HALF] = twc<K*(16 / (2 * HALF))>(csub(a, b));
+            dif_bfly<R, HALF, BASE, K + 1>(u);
+        } else {
+            dif_bfly<R, HALF, BASE + 2 * HALF, 0>(u);
+        }
+    }
+}
+
+template <int R, int HALF>
+__device__ __forceinline__ void dif_stages(float2* u)
+{
+    if constexpr (HALF >= 1) {
+        dif_bfly<R, HALF, 0, 0>(u);
+        dif_stages<R, HALF / 2>(u);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dft_fwd(float2* u)
+{
+    static_assert(R >= 1 && R <= 16 && (R & (R - 1)) == 0, "radix");
+    if constexpr (R > 1) {
+        dif_stages<R, R / 2>(u);
+        constexpr int bits = (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : 4;
+        float2 t[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) t[i] = u[bitrev_small(i, bits)];
+#pragma unroll
+        for (int i = 0; i < R; ++i) u[i] = t[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the job of one axis pass
+// ---------------------------------------------------------------------------
+struct AxisJob {
+    const float2* src;     // input image base
+    float2* dst;           // output image base (may equal src)
+    int64_t es;            // element stride (complex elements)
+    int64_t ls;            // line stride
+    int n;                 // logical line length
+    int count;             // number of lines
+    int inverse;           // 0 forward, 1 inverse
+    float scale;           // 1/n for the inverse, 1 otherwise
+    const float2* chirp;   // Bluestein: exp(-i*pi*k^2/n), k < n
+    const float2* dtab;    // Bluestein: FFT_M(conj chirp embedded) / M
+    const float2* tw;      // twiddle block of this LOGM
+};
+
+// ---------------------------------------------------------------------------
+// FFT of the tile held in registers
+// ---------------------------------------------------------------------------
+template <int LOGM, int THREADS, bool COLS, int P>
+struct FftPass {
+    using C = LineCfg<LOGM, THREADS>;
+    static constexpr int RB = pass_bits(LOGM, P);
+    static constexpr int R = 1 << RB;
+    static constexpr int NB = C::EPT / R;
+    static constexpr int NS = 1 << (4 * P);
+    static constexpr bool LAST = (P == C::NPASS - 1);
+    static constexpr int LST = C::template lstride<COLS>();
+
+    __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, float* re, float* im, const float2* tw)
+    {
+        if constexpr (P > 0) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < C::EPT; ++i) {
+                const int q = pad_pos(t + i * C::TPL);
+                v[i] = make_float2(re[q], im[q]);
+            }
+        }
+        float2 w[C::EPT];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = t + b * C::TPL;
+            const int k = j & (NS - 1);
+            float2 u[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) u[r] = v[b + NB * r];
+            if constexpr (P > 0) {
+                const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * R);
+#pragma unroll
+                for (int r2 = 0; r2 < R / 2; ++r2) {
+                    const float4 q = row[r2];
+                    if (r2 > 0) u[2 * r2] = cmul(u[2 * r2], make_float2(q.x, q.y));
+                    u[2 * r2 + 1] = cmul(u[2 * r2 + 1], make_float2(q.z, q.w));
+                }
+            }
+            dft_fwd<R>(u);
+#pragma unroll
+            for (int r = 0; r < R; ++r) w[b + NB * r] = u[r];
+        }
+        if constexpr (LAST) {
+#pragma unroll
+            for (int i = 0; i < C::EPT; ++i) v[i] = w[i];
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int j = t + b * C::TPL;
+                const int k = j & (NS - 1);
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int q = pad_pos(base + r * NS);
+                    re[q] = w[b + NB * r].x;
+                    im[q] = w[b + NB * r].y;
+                }
+            }
+            FftPass<LOGM, THREADS, COLS, P + 1>::run(v, t, re, im, tw);
+        }
+    }
+};
+
+template <int LOGM, int THREADS, bool COLS>
+struct FftTile {
+    using C = LineCfg<LOGM, THREADS>;
+    __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, int ll, float* lds, const float2* tw)
+    {
+        constexpr int LST = C::template lstride<COLS>();
+        float* re = lds + ll * LST;
+        float* im = lds + C::LPW * LST + ll * LST;
+        FftPass<LOGM, THREADS, COLS, 0>::run(v, t, re, im, tw);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// one tile of one axis pass: load (+chirp), FFT, [x D, FFT], store (+chirp)
+// ---------------------------------------------------------------------------
+template <int LOGM, int THREADS, bool BLUE, bool COLS>
+__device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* lds, int tid)
+{
+    using C = LineCfg<LOGM, THREADS>;
+    const int ll = COLS ? (tid % C::LPW) : (tid / C::TPL);
+    const int t = COLS ? (tid / C::LPW) : (tid % C::TPL);
+    const int line = tile * C::LPW + ll;
+    const bool valid = line < job.count;
+    const int n = job.n;
+    const int64_t es = job.es;
+    const bool inv = job.inverse != 0;
+
+    float2 v[C::EPT];
+    const float2* s = job.src + int64_t(valid ? line : 0) * job.ls;
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int pos = t + i * C::TPL;
+        float2 x = make_float2(0.0f, 0.0f);
+        if (valid && pos < n) {
+            x = s[pos * es];
+            if constexpr (BLUE) {
+                const float2 c = job.chirp[pos];
+                x = inv ? cmulc(x, c) : cmul(x, c);
+            } else {
+                if (inv) x = cconj(x);
+            }
+        }
+        v[i] = x;
+    }
+
+    FftTile<LOGM, THREADS, COLS>::run(v, t, ll, lds, job.tw);
+
+    if constexpr (BLUE) {
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int pos = t + i * C::TPL;
+            const float2 d = job.dtab[pos];
+            v[i] = inv ? cmul(cconj(v[i]), d) : cconj(cmul(v[i], d));
+        }
+        FftTile<LOGM, THREADS, COLS>::run(v, t, ll, lds, job.tw);
+    }
+
+    if (valid) {
+        float2* o = job.dst + int64_t(line) * job.ls;
+        const float sc = job.scale;
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int pos = t + i * C::TPL;
+            if (pos < n) {
+                float2 y = v[i];
+                if constexpr (BLUE) {
+                    const float2 c = job.chirp[pos];
+                    y = inv ? cscale(cconj(cmul(y, c)), sc) : cmul(cconj(y), c);
+                } else {
+                    if (inv) y = cscale(cconj(y), sc);
+                }
+                o[pos * es] = y;
+            }
+        }
+    }
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Give each XCD a contiguous range of tiles so neighbouring column tiles,
+// which share cache lines, meet in the same L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks)
+{
+    constexpr int NXCD = 8;
+    const int xcd = bid % NXCD;
+    const int local = bid / NXCD;
+    const int q = nblocks / NXCD, r = nblocks % NXCD;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+}  // namespace mdft

@@ -1,0 +1,119 @@
+// Plan-time table generation on the device, in FP64 with exactly reduced
+// integer exponents, rounded once to FP32:
+//   * chirp[k]   = exp(-i*pi*(k^2 mod 2n)/n), k < n   (src/gpu/dft/function.cpp:49-71)
+//   * dtab[q]    = (1/M) * sum_l h2[l] exp(-2*pi*i*(l*q mod M)/M), the FFT of the
+//                  circulant embedding h2 of conj(chirp)  (function.cpp:75-92,
+//                  dft.cpp:77-111), with the 1/M of the inverse FFT folded in
+//   * twiddles   = per-pass Stockham rows exp(-2*pi*i*(k*r mod L)/L)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_device.hpp"
+
+namespace mdft {
+
+// exp(-2*pi*i*q/den) for integers 0 <= q < den, den <= 2^62
+__device__ __forceinline__ double2 unit_root(int64_t q, int64_t den)
+{
+    double s, c;
+    sincospi(2.0 * (double)q / (double)den, &s, &c);
+    return make_double2(c, -s);
+}
+
+__global__ void k_gen_twiddle_block(float2* out, int ns, int radix)
+{
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t len = int64_t(ns) * radix;
+    if (idx >= len) return;
+    const int64_t k = idx / radix, r = idx % radix;
+    const double2 w = unit_root((k * r) % len, len);
+    out[idx] = make_float2((float)w.x, (float)w.y);
+}
+
+__global__ void k_gen_chirp(double2* chirp64, float2* chirp32, int n)
+{
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t two_n = 2 * int64_t(n);
+    // k^2 mod 2n without overflow for k < 2^31: reduce k first
+    const int64_t kr = k % two_n;
+    const int64_t q = (int64_t)(((unsigned __int128)kr * kr) % (unsigned __int128)two_n);
+    const double2 c = unit_root(q, two_n);
+    chirp64[k] = c;
+    chirp32[k] = make_float2((float)c.x, (float)c.y);
+}
+
+// cos(2*pi*q/m), q < m
+__global__ void k_gen_cos(double* cosm, int m)
+{
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    cosm[q] = cospi(2.0 * (double)q / (double)m);
+}
+
+// The circulant kernel is even (h2[l] == h2[m-l]), so its DFT is
+//   D[q] = h2[0] + sum_{0<l<n} f_l * h2[l] * cos(2*pi*l*q/m),
+// f_l = 2 except f = 1 for l == m/2 (its own mirror when m == 2n-2).
+// h2[l] = conj(chirp[l]).  One workgroup per output, lanes split the sum.
+__global__ void __launch_bounds__(256) k_gen_dtab(float2* dtab, const double2* chirp64, const double* cosm, int n,
+                                                  int m)
+{
+    const int q = blockIdx.x;
+    double sr = 0.0, si = 0.0;
+    for (int l = 1 + threadIdx.x; l < n; l += blockDim.x) {
+        const double f = (2 * l == m) ? 1.0 : 2.0;
+        const int64_t idx = (int64_t(l) * q) % m;
+        const double c = cosm[idx] * f;
+        sr += chirp64[l].x * c;
+        si -= chirp64[l].y * c;
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = si;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double inv_m = 1.0 / (double)m;
+        const double re = (red[0][0] + chirp64[0].x) * inv_m;
+        const double im = (red[1][0] - chirp64[0].y) * inv_m;
+        dtab[q] = make_float2((float)re, (float)im);
+    }
+}
+
+// --- host launchers --------------------------------------------------------
+
+hipError_t gen_twiddles(float2* table, hipStream_t stream)
+{
+    for (int logm = 1; logm <= MAX_LOGM; ++logm) {
+        for (int p = 1; p < npass_of(logm); ++p) {
+            const int ns = 1 << (4 * p);
+            const int radix = 1 << pass_bits(logm, p);
+            const int64_t len = int64_t(ns) * radix;
+            float2* out = table + tw_off(logm) + tw_pass_off(logm, p);
+            const int blocks = (int)((len + 255) / 256);
+            hipLaunchKernelGGL(k_gen_twiddle_block, dim3(blocks), dim3(256), 0, stream, out, ns, radix);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t gen_chirp(double2* chirp64, float2* chirp32, int n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_gen_chirp, dim3((n + 255) / 256), dim3(256), 0, stream, chirp64, chirp32, n);
+    return hipGetLastError();
+}
+
+hipError_t gen_dtab(float2* dtab, const double2* chirp64, double* cos_scratch, int n, int m, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_gen_cos, dim3((m + 255) / 256), dim3(256), 0, stream, cos_scratch, m);
+    hipLaunchKernelGGL(k_gen_dtab, dim3(m), dim3(256), 0, stream, dtab, chirp64, (const double*)cos_scratch, n, m);
+    return hipGetLastError();
+}
+
+}  // namespace mdft

@@ -1,0 +1,207 @@
+"""Python bindings (ctypes) over the C ABI of libmdft.so (include/mdft.h).
+
+Mirrors the reference's interfaces for the 2-D DFT path:
+  * ComputeVector.create_buffers(width, height) / exec(inverse, data)
+    (src/gpu/dft/compute.h:52-60, compute.cpp:229-283),
+  * create_hip(x, y).exec(inverse, data), the ns::dft::DFT shape
+    (src/dft/dft.h:25-31, src/dft/fftw.cpp:178-186),
+plus device-side entry points (Plan.exec_device, Batch) that take raw device
+pointers and a HIP stream handle (e.g. a torch tensor's data_ptr() and
+torch.cuda.current_stream().cuda_stream).
+
+The library is required: importing this module raises if libmdft.so is not
+built, and there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmdft.so")
+
+MDFT_OK = 0
+MDFT_ERR_INVALID = -1
+MDFT_ERR_SIZE = -2
+MDFT_ERR_NO_BUFFERS = -3
+MDFT_ERR_DATA_SIZE = -4
+MDFT_ERR_DEVICE = -5
+MDFT_ERR_BUDGET = -6
+MDFT_ERR_TIMEOUT = -7
+MDFT_ERR_STATE = -8
+
+MDFT_BATCH_GRAPH = 1
+MDFT_BATCH_TWO_PHASE = 2
+
+
+class ErrorException(RuntimeError):
+    """Raised with the library's message text (the reference's ns::ErrorException)."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class mdft_desc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("inverse", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32),
+        ("src_offset", ctypes.c_uint64),
+        ("dst_offset", ctypes.c_uint64),
+    ]
+
+
+_vp = ctypes.c_void_p
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libmdft.so is not built ({LIB_PATH}); run `python -m math_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "mdft_last_error": ([], ctypes.c_char_p),
+            "mdft_version": ([], ctypes.c_char_p),
+            "mdft_init": ([ctypes.c_int], ctypes.c_int),
+            "mdft_plan_create": ([ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int),
+            "mdft_plan_destroy": ([_vp], ctypes.c_int),
+            "mdft_plan_info": ([_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+            "mdft_exec": ([_vp, ctypes.c_int, _vp, ctypes.c_size_t], ctypes.c_int),
+            "mdft_exec_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+            "mdft_batch_create": ([ctypes.POINTER(_vp), ctypes.POINTER(mdft_desc), ctypes.c_uint32, ctypes.c_uint32],
+                                  ctypes.c_int),
+            "mdft_batch_exec_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+            "mdft_batch_check": ([_vp], ctypes.c_int),
+            "mdft_batch_destroy": ([_vp], ctypes.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _L = L
+    return _L
+
+
+def _check(rc):
+    if rc != MDFT_OK:
+        msg = lib().mdft_last_error().decode()
+        raise ErrorException(rc, msg)
+
+
+def init(device=0):
+    _check(lib().mdft_init(device))
+
+
+class Plan:
+    """One width x height transform (mdft_plan_create)."""
+
+    def __init__(self, width, height):
+        self._h = _vp()
+        _check(lib().mdft_plan_create(ctypes.byref(self._h), int(width), int(height)))
+        self.width, self.height = int(width), int(height)
+
+    def close(self):
+        if self._h:
+            lib().mdft_plan_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        m1, m2, b = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        _check(lib().mdft_plan_info(self._h, ctypes.byref(m1), ctypes.byref(m2), ctypes.byref(b)))
+        return m1.value, m2.value, b.value
+
+    def exec(self, inverse, data):
+        """In place on a C-contiguous complex64 numpy array of width*height values."""
+        if not isinstance(data, np.ndarray) or data.dtype != np.complex64 or not data.flags.c_contiguous:
+            raise ErrorException(MDFT_ERR_DATA_SIZE, "Wrong DFT buffer size")
+        _check(lib().mdft_exec(self._h, int(bool(inverse)), data.ctypes.data, data.size))
+
+    def exec_device(self, inverse, src_ptr, dst_ptr, stream=0):
+        _check(lib().mdft_exec_device(self._h, int(bool(inverse)), src_ptr, dst_ptr, stream))
+
+
+class ComputeVector:
+    """Shape of ns::gpu::dft::ComputeVector (src/gpu/dft/compute.h:52-60)."""
+
+    def __init__(self):
+        self._plan = None
+
+    def create_buffers(self, width, height):
+        if self._plan is not None:
+            self._plan.close()
+            self._plan = None
+        self._plan = Plan(width, height)
+
+    def exec(self, inverse, data):
+        if self._plan is None:
+            _check_null_plan_exec(inverse, data)
+        self._plan.exec(inverse, data)
+
+
+def _check_null_plan_exec(inverse, data):
+    # the library raises "No DFT buffers" for a NULL plan (compute.cpp:259-262)
+    p = data.ctypes.data if isinstance(data, np.ndarray) else None
+    n = data.size if isinstance(data, np.ndarray) else 0
+    _check(lib().mdft_exec(None, int(bool(inverse)), p, n))
+
+
+def create_compute_vector():
+    """create_compute_vector(DeviceSearchType) (src/gpu/dft/compute.h:68)."""
+    return ComputeVector()
+
+
+class DFT:
+    """Shape of ns::dft::DFT with a fixed size (src/dft/dft.h:25-31)."""
+
+    def __init__(self, x, y):
+        if not (x > 0 and y > 0):
+            raise ErrorException(MDFT_ERR_SIZE, "Error HIP data size")
+        self._plan = Plan(x, y)
+
+    def exec(self, inverse, data):
+        self._plan.exec(inverse, data)
+
+
+def create_hip(x, y):
+    """The HIP counterpart of create_fftw(x, y) / create_cufft(x, y) (src/dft/fftw.h:28)."""
+    return DFT(x, y)
+
+
+class Batch:
+    """Many transforms of mixed sizes in one persistent launch (mdft_batch_create)."""
+
+    def __init__(self, sizes, inverse, src_offsets, dst_offsets, flags=0):
+        n = len(sizes)
+        arr = (mdft_desc * max(n, 1))()
+        for i, ((w, h), inv, so, do) in enumerate(zip(sizes, inverse, src_offsets, dst_offsets)):
+            arr[i] = mdft_desc(int(w), int(h), int(bool(inv)), 0, int(so), int(do))
+        self._h = _vp()
+        _check(lib().mdft_batch_create(ctypes.byref(self._h), arr, n, flags))
+
+    def exec_device(self, src_ptr, dst_ptr, stream=0):
+        _check(lib().mdft_batch_exec_device(self._h, src_ptr, dst_ptr, stream))
+
+    def check(self):
+        _check(lib().mdft_batch_check(self._h))
+
+    def close(self):
+        if self._h:
+            lib().mdft_batch_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -87,6 +87,17 @@ int mdft_exec(mdft_plan plan, int inverse, void* data, size_t count);
  * caller's command buffer (src/gpu/dft/dft.h:41, dft.cpp:326-347). */
 int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, void* stream);
 
+/* ---- tile-shape tuning ---------------------------------------------------- */
+
+/* Tile shape the single-image kernels use for lines of padded length 2^logm
+ * (1 <= logm <= 14), Bluestein (blue != 0) or power-of-two, along rows
+ * (cols == 0) or columns: 0 = 256-thread tiles, 1 = 1024-thread tiles (four
+ * times the lines per tile).  No counterpart in the reference, whose tile
+ * size follows the device's shared-memory limit (src/gpu/dft/fft.cpp:51-61,
+ * function.cpp:94-118); this build picks it per shape by measurement. */
+int mdft_tune_get(int logm, int blue, int cols, int* variant);
+int mdft_tune_set(int logm, int blue, int cols, int variant);
+
 /* ---- many transforms of mixed sizes in one persistent launch ------------- */
 
 typedef struct {
@@ -100,16 +111,19 @@ typedef struct {
 
 typedef struct mdft_batch_s* mdft_batch;
 
-#define MDFT_BATCH_GRAPH 1u       /* replay the launch as a captured hipGraph */
-#define MDFT_BATCH_TWO_PHASE 2u   /* rows and columns as two launches instead of dependency counters */
+#define MDFT_BATCH_GRAPH 1u       /* replay the launches as a captured hipGraph */
+#define MDFT_BATCH_PERSISTENT 2u  /* persistent workgroups taking items from a ticket counter */
+#define MDFT_BATCH_DEPENDENT 4u   /* one persistent launch; column tiles wait on per-image row counters */
 
 /* Plan a batch of transforms (no counterpart in the reference, whose exec
  * handles one image per call).  Every side must have a padded length
  * compute_m(side) <= 8192: powers of two up to 8192, other sides up to 4097. */
 int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count, uint32_t flags);
 
-/* Enqueue the whole batch on a stream: one persistent kernel (plus a counter
- * reset) for all transforms.  src may equal dst (in place per transform). */
+/* Enqueue the whole batch on a stream.  Default shape: two launches (small
+ * images whole plus row tiles, then column tiles), one workgroup per item;
+ * the flags select persistent or dependency-counter shapes.  src may equal
+ * dst (in place per transform). */
 int mdft_batch_exec_device(mdft_batch batch, const void* src, void* dst, void* stream);
 
 /* Device error flag of the last completed launch: 0, or MDFT_ERR_TIMEOUT if

@@ -19,12 +19,15 @@ COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions stay off on this target
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # roundings are fixed by explicit fmas in the source (bitwise-reproducible lines)
+    "-ffp-contract=off",
     "-Wall", "-Wno-unused-function",
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 
 SOURCES = [
     "kernels/axis.hip",
+    "kernels/batch.hip",
     "kernels/tables.hip",
     "host/runtime.cpp",
     "host/capi.cpp",

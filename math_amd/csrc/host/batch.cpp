@@ -1,15 +1,16 @@
-// Batched transforms of mixed sizes in one persistent launch.
+// Batched transforms of mixed sizes.
 //
-// Each transform contributes a row job and a column job; each job is cut
-// into tiles of whole lines.  The work list interleaves images, biggest
-// first (rows of image 0, rows of image 1, columns of image 0, rows of image
-// 2, ...), so a column job starts soon after its rows and finds them in L2 or
-// the 256 MB MALL.  A line length gets the same tile shape in every batch,
-// so a result is bitwise the same alone or in any mix.
+// Each image contributes a row job and a column job, cut into tiles of whole
+// lines.  A small image whose rows and columns each fit one tile becomes a
+// single "whole" item done by one workgroup.  Items are ordered by kernel
+// variant (padded length, Bluestein or not, axis) so neighbouring workgroups
+// run the same code.  A line length gets the same radix schedule in every
+// batch, so a result is bitwise the same alone or in any mix.
 #include "capi_util.hpp"
 
 #include <algorithm>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 using mdft::BatchJob;
@@ -20,47 +21,53 @@ using mdft::LinePlan;
 struct mdft_batch_s {
     int device = 0;
     uint32_t flags = 0;
-    std::vector<mdft_desc> descs;
+    int threads = 256;
     std::vector<std::shared_ptr<const LinePlan>> keep;
     // host copy of the jobs with offsets, patched with base pointers at exec
     std::vector<BatchJob> jobs;
     std::vector<uint64_t> job_src_off, job_dst_off;
-    std::vector<int> job_src_is_dst;  // column job reading the row job's output
+    std::vector<int> job_src_is_dst;                    // column job reading the row job's output
     std::vector<std::pair<uint64_t, uint64_t>> copies;  // 1x1 identities (src_off, dst_off)
-    uint32_t nitems = 0, nrow_items = 0;
+    uint32_t nphase1 = 0, nitems = 0;
     uint32_t ncounters = 0;
     BatchJob* jobs_dev = nullptr;
     BatchJob* jobs_pinned = nullptr;
     uint2* items_dev = nullptr;
-    uint32_t* ctl_dev = nullptr;  // two control blocks (two-phase uses both)
+    uint32_t* ctl_dev = nullptr;  // two control blocks of nctl words
     uint32_t nctl = 0;
-    int grid = 0;
+    int grid = 0;  // persistent grid
     const void* cur_src = nullptr;
     void* cur_dst = nullptr;
     hipStream_t last_stream = nullptr;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    uint32_t spin_limit = 1u << 26;
+    uint32_t spin_limit = 1u << 20;  // fail-safe of the dependency wait: then the error flag
 };
 
 namespace {
 
 void enqueue_batch(mdft_batch_s* b, hipStream_t stream)
 {
-    if (b->flags & MDFT_BATCH_TWO_PHASE) {
-        if (b->nrow_items)
-            mdft::hip_check(mdft::launch_batch(b->jobs_dev, b->items_dev, b->nrow_items, b->ctl_dev, b->nctl, b->grid,
-                                               b->spin_limit, stream),
-                            "batch launch (rows)");
-        if (b->nitems > b->nrow_items)
-            mdft::hip_check(mdft::launch_batch(b->jobs_dev, b->items_dev + b->nrow_items, b->nitems - b->nrow_items,
-                                               b->ctl_dev + b->nctl, b->nctl, b->grid, b->spin_limit, stream),
-                            "batch launch (columns)");
-    } else if (b->nitems) {
-        mdft::hip_check(mdft::launch_batch(b->jobs_dev, b->items_dev, b->nitems, b->ctl_dev, b->nctl, b->grid,
-                                           b->spin_limit, stream),
+    const int T = b->threads;
+    if (b->flags & MDFT_BATCH_DEPENDENT) {
+        mdft::hip_check(mdft::launch_batch_persistent(T, b->jobs_dev, b->items_dev, b->nitems, b->ctl_dev, b->nctl,
+                                                      b->grid, b->spin_limit, stream),
                         "batch launch");
+    } else if (b->flags & MDFT_BATCH_PERSISTENT) {
+        mdft::hip_check(mdft::launch_batch_persistent(T, b->jobs_dev, b->items_dev, b->nphase1, b->ctl_dev, b->nctl,
+                                                      b->grid, b->spin_limit, stream),
+                        "batch launch (phase 1)");
+        mdft::hip_check(mdft::launch_batch_persistent(T, b->jobs_dev, b->items_dev + b->nphase1,
+                                                      b->nitems - b->nphase1, b->ctl_dev + b->nctl, b->nctl, b->grid,
+                                                      b->spin_limit, stream),
+                        "batch launch (phase 2)");
+    } else {
+        mdft::hip_check(mdft::launch_batch_grid(T, b->jobs_dev, b->items_dev, b->nphase1, stream),
+                        "batch launch (phase 1)");
+        mdft::hip_check(mdft::launch_batch_grid(T, b->jobs_dev, b->items_dev + b->nphase1, b->nitems - b->nphase1,
+                                                stream),
+                        "batch launch (phase 2)");
     }
 }
 
@@ -74,57 +81,76 @@ int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count,
     if (!batch) throw Error(MDFT_ERR_INVALID, "mdft_batch_create: batch is NULL");
     *batch = nullptr;
     if (!descs && count) throw Error(MDFT_ERR_INVALID, "mdft_batch_create: descs is NULL");
+    const bool dependent = (flags & MDFT_BATCH_DEPENDENT) != 0;
+
+    // validate sizes and pick the workgroup size before touching the device
+    int maxlogm = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const int m1 = mdft::compute_m(descs[i].width), m2 = mdft::compute_m(descs[i].height);
+        maxlogm = std::max({maxlogm, mdft::ilog2(m1), mdft::ilog2(m2)});
+    }
+    const int limit = mdft::batch_max_logm(512);
+    if (maxlogm > limit) {
+        for (uint32_t i = 0; i < count; ++i) {
+            const mdft_desc& d = descs[i];
+            if (mdft::ilog2(mdft::compute_m(d.width)) > limit || mdft::ilog2(mdft::compute_m(d.height)) > limit)
+                throw Error(MDFT_ERR_SIZE, "batch transform " + std::to_string(i) + " (" + std::to_string(d.width) +
+                                               "x" + std::to_string(d.height) +
+                                               ") exceeds the batch padded-length limit " +
+                                               std::to_string(1 << limit));
+        }
+    }
+    const int T = maxlogm > mdft::batch_max_logm(256) ? 512 : 256;
+
     Context& ctx = Context::get(Context::current_device());
     mdft::DeviceGuard g(ctx.device());
     auto b = std::make_unique<mdft_batch_s>();
     b->device = ctx.device();
     b->flags = flags;
-    b->descs.assign(descs, descs + count);
-    const int maxlogm = mdft::batch_max_logm();
+    b->threads = T;
 
-    struct Img {
-        uint32_t idx;
-        double work;
+    std::vector<uint32_t> order(count);
+    std::iota(order.begin(), order.end(), 0u);
+    auto work = [&](uint32_t i) {
+        return (double)mdft::compute_m(descs[i].width) * descs[i].height +
+               (double)mdft::compute_m(descs[i].height) * descs[i].width;
     };
-    std::vector<Img> order;
-    for (uint32_t i = 0; i < count; ++i) {
-        const mdft_desc& d = descs[i];
-        const int m1 = mdft::compute_m(d.width), m2 = mdft::compute_m(d.height);
-        if (mdft::ilog2(m1) > maxlogm || mdft::ilog2(m2) > maxlogm)
-            throw Error(MDFT_ERR_SIZE, "batch transform " + std::to_string(i) + " (" + std::to_string(d.width) + "x" +
-                                           std::to_string(d.height) + ") exceeds the batch padded-length limit " +
-                                           std::to_string(1 << maxlogm));
-        order.push_back({i, (double)m1 * d.height + (double)m2 * d.width});
-    }
-    std::stable_sort(order.begin(), order.end(), [](const Img& a, const Img& c) { return a.work > c.work; });
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return work(a) > work(c); });
 
-    struct JobTiles {
-        int job;
-        int ntiles;
+    // item with its ordering key (variant first, then image order)
+    struct Item {
+        int phase;
+        int logm, blue, kind;  // kind: 0 whole, 1 row tile, 2 column tile
+        uint32_t seq;
+        uint2 v;
     };
-    std::vector<JobTiles> rows, cols;  // per image in order (job -1 if absent)
-    uint32_t counter = 0;
-    for (const Img& im : order) {
-        const mdft_desc& d = descs[im.idx];
+    std::vector<Item> items;
+    uint32_t counter = 0, seq = 0;
+    std::vector<std::pair<std::vector<uint2>, std::vector<uint2>>> dep_lists;  // per image: row items, col items
+    for (uint32_t idx : order) {
+        const mdft_desc& d = descs[idx];
         const int w = (int)d.width, h = (int)d.height;
         auto rp = ctx.line_plan(w);
         auto cp = ctx.line_plan(h);
         b->keep.push_back(rp);
         b->keep.push_back(cp);
-        JobTiles r{-1, 0}, c{-1, 0};
-        const int sig = (w > 1 && h > 1 && !(flags & MDFT_BATCH_TWO_PHASE)) ? (int)counter++ : -1;
+        const int lpw_r = w > 1 ? mdft::batch_lines_per_tile(T, rp->logm) : 0;
+        const int lpw_c = h > 1 ? mdft::batch_lines_per_tile(T, cp->logm) : 0;
+        const bool whole = (w > 1 && h > 1 && h <= lpw_r && w <= lpw_c);
+        const int ntr = w > 1 ? (h + lpw_r - 1) / lpw_r : 0;
+        const int ntc = h > 1 ? (w + lpw_c - 1) / lpw_c : 0;
+        const int sig = (dependent && w > 1 && h > 1 && !whole) ? (int)counter++ : -1;
+        int rj = -1, cj = -1;
         if (w > 1) {
             BatchJob j{};
             j.job = ctx.make_job(*rp, nullptr, nullptr, 1, w, h, d.inverse != 0);
             j.logm = rp->logm;
             j.blue = rp->blue;
             j.cols = 0;
-            const int lpw = mdft::batch_lines_per_tile(rp->logm);
-            j.ntiles = (h + lpw - 1) / lpw;
+            j.ntiles = ntr;
             j.dep = -1;
-            j.dep_need = 0;
             j.sig = sig;
-            r = {(int)b->jobs.size(), j.ntiles};
+            rj = (int)b->jobs.size();
             b->jobs.push_back(j);
             b->job_src_off.push_back(d.src_offset);
             b->job_dst_off.push_back(d.dst_offset);
@@ -136,56 +162,69 @@ int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count,
             j.logm = cp->logm;
             j.blue = cp->blue;
             j.cols = 1;
-            const int lpw = mdft::batch_lines_per_tile(cp->logm);
-            j.ntiles = (w + lpw - 1) / lpw;
-            j.dep = (w > 1) ? sig : -1;
-            j.dep_need = (w > 1) ? r.ntiles : 0;
+            j.ntiles = ntc;
+            j.dep = sig;
+            j.dep_need = ntr;
             j.sig = -1;
-            c = {(int)b->jobs.size(), j.ntiles};
+            cj = (int)b->jobs.size();
             b->jobs.push_back(j);
             b->job_src_off.push_back(w > 1 ? d.dst_offset : d.src_offset);
             b->job_dst_off.push_back(d.dst_offset);
             b->job_src_is_dst.push_back(w > 1 ? 1 : 0);
         }
         if (w == 1 && h == 1) b->copies.push_back({d.src_offset, d.dst_offset});
-        rows.push_back(r);
-        cols.push_back(c);
+        std::vector<uint2> ri, ci;
+        if (whole) {
+            items.push_back({1, rp->logm, rp->blue, 0, seq++, make_uint2(rj, mdft::BATCH_WHOLE | (uint32_t)cj)});
+            ri.push_back(items.back().v);
+        } else {
+            for (int t = 0; t < ntr; ++t) {
+                items.push_back({1, rp->logm, rp->blue, 1, seq++, make_uint2(rj, t)});
+                ri.push_back(items.back().v);
+            }
+            for (int t = 0; t < ntc; ++t) {
+                items.push_back({w > 1 ? 2 : 1, cp->logm, cp->blue, 2, seq++, make_uint2(cj, t)});
+                ci.push_back(items.back().v);
+            }
+        }
+        dep_lists.push_back({ri, ci});
     }
 
-    // work list
-    std::vector<uint2> items;
-    auto add = [&](const JobTiles& jt) {
-        for (int t = 0; t < jt.ntiles; ++t) items.push_back(make_uint2((uint32_t)jt.job, (uint32_t)t));
-    };
-    if (flags & MDFT_BATCH_TWO_PHASE) {
-        for (auto& r : rows) add(r);
-        b->nrow_items = (uint32_t)items.size();
-        for (auto& c : cols) add(c);
-    } else {
-        // rows of k+1 before columns of k
-        for (size_t k = 0; k < rows.size(); ++k) {
-            add(rows[k]);
-            if (k > 0) add(cols[k - 1]);
+    std::vector<uint2> list;
+    if (dependent) {
+        // rows of image k+1 before columns of image k: a column tile only waits
+        // on row tiles with smaller tickets
+        for (size_t k = 0; k < dep_lists.size(); ++k) {
+            list.insert(list.end(), dep_lists[k].first.begin(), dep_lists[k].first.end());
+            if (k > 0) list.insert(list.end(), dep_lists[k - 1].second.begin(), dep_lists[k - 1].second.end());
         }
-        if (!cols.empty()) add(cols.back());
-        b->nrow_items = 0;
+        if (!dep_lists.empty()) list.insert(list.end(), dep_lists.back().second.begin(), dep_lists.back().second.end());
+        b->nphase1 = (uint32_t)list.size();
+    } else {
+        std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) {
+            return std::make_tuple(a.phase, -a.logm, a.blue, a.kind, a.seq) <
+                   std::make_tuple(c.phase, -c.logm, c.blue, c.kind, c.seq);
+        });
+        for (const Item& it : items) {
+            if (it.phase == 1) ++b->nphase1;
+            list.push_back(it.v);
+        }
     }
-    b->nitems = (uint32_t)items.size();
+    b->nitems = (uint32_t)list.size();
     b->ncounters = counter;
     b->nctl = mdft::BATCH_CTL_HEADER + counter;
 
     int per_cu = 0;
-    mdft::hip_check(mdft::batch_occupancy(&per_cu), "occupancy");
-    if (per_cu < 1) per_cu = 1;
-    b->grid = std::max(1, std::min<int>(ctx.cu_count() * per_cu, (int)std::max<uint32_t>(b->nitems, 1)));
+    mdft::hip_check(mdft::batch_occupancy(T, &per_cu), "occupancy");
+    b->grid = std::max(1, ctx.cu_count() * std::max(per_cu, 1));
 
     const size_t jb = sizeof(BatchJob) * std::max<size_t>(b->jobs.size(), 1);
     mdft::hip_check(hipMalloc(&b->jobs_dev, jb), "hipMalloc jobs");
     mdft::hip_check(hipHostMalloc(&b->jobs_pinned, jb, hipHostMallocDefault), "hipHostMalloc jobs");
-    mdft::hip_check(hipMalloc(&b->items_dev, sizeof(uint2) * std::max<size_t>(items.size(), 1)), "hipMalloc items");
+    mdft::hip_check(hipMalloc(&b->items_dev, sizeof(uint2) * std::max<size_t>(list.size(), 1)), "hipMalloc items");
     mdft::hip_check(hipMalloc(&b->ctl_dev, sizeof(uint32_t) * 2 * b->nctl), "hipMalloc ctl");
-    if (!items.empty())
-        mdft::hip_check(hipMemcpy(b->items_dev, items.data(), sizeof(uint2) * items.size(), hipMemcpyHostToDevice),
+    if (!list.empty())
+        mdft::hip_check(hipMemcpy(b->items_dev, list.data(), sizeof(uint2) * list.size(), hipMemcpyHostToDevice),
                         "upload items");
     mdft::hip_check(hipMemset(b->ctl_dev, 0, sizeof(uint32_t) * 2 * b->nctl), "clear ctl");
 
@@ -208,8 +247,8 @@ int mdft_batch_exec_device(mdft_batch b, const void* src, void* dst, void* strea
     mdft::DeviceGuard g(b->device);
     hipStream_t stream = static_cast<hipStream_t>(stream_p);
     if (src != b->cur_src || dst != b->cur_dst) {
-        // the pinned staging copy may still be in flight from an earlier patch
-        if (b->last_stream || b->cur_src) mdft::hip_check(hipDeviceSynchronize(), "synchronize before re-patch");
+        // the pinned staging copy of an earlier patch may still be in flight
+        if (b->cur_src) mdft::hip_check(hipDeviceSynchronize(), "synchronize before re-patch");
         const float2* s = static_cast<const float2*>(src);
         float2* d = static_cast<float2*>(dst);
         for (size_t i = 0; i < b->jobs.size(); ++i) {

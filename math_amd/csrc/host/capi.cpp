@@ -117,4 +117,28 @@ int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, vo
     MDFT_API_END
 }
 
+static void check_tune_args(int logm, int blue, int cols)
+{
+    if (logm < 1 || logm > mdft::MAX_LOGM || (blue != 0 && blue != 1) || (cols != 0 && cols != 1))
+        throw Error(MDFT_ERR_INVALID, "tune: logm must be in [1, 14], blue and cols 0 or 1");
+}
+
+int mdft_tune_get(int logm, int blue, int cols, int* variant)
+{
+    MDFT_API_BEGIN
+    check_tune_args(logm, blue, cols);
+    if (!variant) throw Error(MDFT_ERR_INVALID, "tune: variant is NULL");
+    *variant = Context::get(Context::current_device()).variant(logm, blue != 0, cols != 0);
+    MDFT_API_END
+}
+
+int mdft_tune_set(int logm, int blue, int cols, int variant)
+{
+    MDFT_API_BEGIN
+    check_tune_args(logm, blue, cols);
+    if (variant != 0 && variant != 1) throw Error(MDFT_ERR_INVALID, "tune: variant must be 0 or 1");
+    Context::get(Context::current_device()).set_variant(logm, blue != 0, cols != 0, variant);
+    MDFT_API_END
+}
+
 }  // extern "C"

@@ -87,6 +87,9 @@ Context::Context(int device) : device_(device)
     hip_check(hipMalloc(&twiddles_, sizeof(float2) * (size_t)tw_total()), "hipMalloc twiddles");
     hip_check(gen_twiddles(twiddles_, gen_stream_), "twiddle generation");
     hip_check(hipStreamSynchronize(gen_stream_), "twiddle generation");
+    // default tile shapes: wide tiles for columns of 1024 points and more
+    for (int l = 0; l <= MAX_LOGM; ++l)
+        for (int b = 0; b < 2; ++b) variant_[l][b][1] = (l >= 10) ? 1 : 0;
 }
 
 std::shared_ptr<const LinePlan> Context::line_plan(int n)
@@ -155,7 +158,8 @@ void Context::enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, i
                            bool inverse, bool cols, hipStream_t stream) const
 {
     const AxisJob j = make_job(lp, src, dst, es, ls, count, inverse);
-    hip_check(launch_axis(lp.logm, j, lp.blue, cols, stream), "axis pass launch");
+    hip_check(launch_axis(lp.logm, j, lp.blue, cols, variant(lp.logm, lp.blue, cols) != 0, stream),
+              "axis pass launch");
 }
 
 void Context::enqueue_transform(const LinePlan& row, const LinePlan& col, int w, int h, bool inverse,

@@ -70,6 +70,11 @@ public:
     AxisJob make_job(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
                      bool inverse) const;
 
+    // Tile-shape table of the single-image kernels: variant 0 = narrow
+    // (256-thread) tiles, 1 = wide (1024-thread) tiles, per (logm, blue, cols).
+    int variant(int logm, bool blue, bool cols) const { return variant_[logm][blue][cols]; }
+    void set_variant(int logm, bool blue, bool cols, int v) { variant_[logm][blue][cols] = (uint8_t)v; }
+
 private:
     explicit Context(int device);
     int device_;
@@ -80,6 +85,7 @@ private:
     size_t scratch64_doubles_ = 0;
     std::mutex mu_;
     std::unordered_map<int, std::shared_ptr<const LinePlan>> lines_;
+    uint8_t variant_[MAX_LOGM + 1][2][2] = {};
 };
 
 // RAII guard that makes a device current for the calling thread.

@@ -34,14 +34,17 @@ namespace mdft {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// Products are written with explicit fused multiply-adds and the library is
+// built with -ffp-contract=off, so every rounding is fixed by the source and a
+// line gives bitwise the same result in every kernel that runs it.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
 }
 // a * conj(b)
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+    return make_float2(__builtin_fmaf(a.x, b.x, a.y * b.y), __builtin_fmaf(a.y, b.x, -(a.x * b.y)));
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
@@ -93,16 +96,17 @@ struct LineCfg {
     static_assert(TPL <= THREADS, "line does not fit the workgroup");
     static_assert(LPW * TPL == THREADS, "workgroup must hold whole lines");
     static constexpr int PADM = M + (M >> 4);
-    // line stride in LDS words, offset so the lines of one wave land on
-    // distinct banks for both thread mappings
-    static constexpr int want_rows = TPL >= 32 ? 0 : TPL;
-    static constexpr int want_cols = LPW >= 32 ? 1 : 32 / LPW;
+    // Line stride in LDS words.  Lanes of one wave cover q consecutive
+    // positions of 32/q lines (rows: q = TPL; columns, lines fastest:
+    // q = 32/LPW); a stride of q times an odd number puts them on distinct
+    // banks.
     template <bool COLS>
     static constexpr int lstride()
     {
-        const int want = COLS ? want_cols : want_rows;
+        const int q = COLS ? (LPW >= 32 ? 1 : 32 / LPW) : (TPL >= 32 ? 0 : TPL);
+        if (q == 0) return PADM;
         int s = PADM;
-        while ((s & 31) != (want & 31)) ++s;
+        while (s % (2 * q) != q) ++s;
         return s;
     }
     template <bool COLS>
@@ -165,7 +169,7 @@ __device__ __forceinline__ float2 twc(float2 d)
         return make_float2(r * (d.y - d.x), -r * (d.x + d.y));
     } else {
         const float c = kC16[m], s = kS16[m];
-        return make_float2(d.x * c + d.y * s, d.y * c - d.x * s);
+        return make_float2(__builtin_fmaf(d.x, c, d.y * s), __builtin_fmaf(d.y, c, -(d.x * s)));
     }
 }
 
@@ -259,7 +263,7 @@ struct FftPass {
                 v[i] = make_float2(re[q], im[q]);
             }
         }
-        float2 w[C::EPT];
+        // butterfly b reads and writes only the slots b + NB*r: update in place
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const int j = t + b * C::TPL;
@@ -278,12 +282,9 @@ struct FftPass {
             }
             dft_fwd<R>(u);
 #pragma unroll
-            for (int r = 0; r < R; ++r) w[b + NB * r] = u[r];
+            for (int r = 0; r < R; ++r) v[b + NB * r] = u[r];
         }
-        if constexpr (LAST) {
-#pragma unroll
-            for (int i = 0; i < C::EPT; ++i) v[i] = w[i];
-        } else {
+        if constexpr (!LAST) {
             __syncthreads();
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -293,8 +294,8 @@ struct FftPass {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int q = pad_pos(base + r * NS);
-                    re[q] = w[b + NB * r].x;
-                    im[q] = w[b + NB * r].y;
+                    re[q] = v[b + NB * r].x;
+                    im[q] = v[b + NB * r].y;
                 }
             }
             FftPass<LOGM, THREADS, COLS, P + 1>::run(v, t, re, im, tw);

@@ -31,7 +31,8 @@ MDFT_ERR_TIMEOUT = -7
 MDFT_ERR_STATE = -8
 
 MDFT_BATCH_GRAPH = 1
-MDFT_BATCH_TWO_PHASE = 2
+MDFT_BATCH_PERSISTENT = 2
+MDFT_BATCH_DEPENDENT = 4
 
 
 class ErrorException(RuntimeError):
@@ -57,9 +58,21 @@ _vp = ctypes.c_void_p
 _L = None
 
 
+def _share_runtime_with_torch():
+    # libmdft.so needs libamdhip64.so.7.  torch's HIP library asks for its own
+    # bundled copy by the unversioned name, so if libmdft.so loaded first the
+    # process would end up with two HIP runtimes and torch would see no GPU.
+    # Loading torch first makes libmdft.so bind to the runtime already loaded.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     global _L
     if _L is None:
+        _share_runtime_with_torch()
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"libmdft.so is not built ({LIB_PATH}); run `python -m math_amd.build`")
         L = ctypes.CDLL(LIB_PATH)
@@ -73,6 +86,8 @@ def lib():
                                 ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "mdft_exec": ([_vp, ctypes.c_int, _vp, ctypes.c_size_t], ctypes.c_int),
             "mdft_exec_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+            "mdft_tune_get": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+            "mdft_tune_set": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
             "mdft_batch_create": ([ctypes.POINTER(_vp), ctypes.POINTER(mdft_desc), ctypes.c_uint32, ctypes.c_uint32],
                                   ctypes.c_int),
             "mdft_batch_exec_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
@@ -95,6 +110,16 @@ def _check(rc):
 
 def init(device=0):
     _check(lib().mdft_init(device))
+
+
+def tune_get(logm, blue, cols):
+    v = ctypes.c_int()
+    _check(lib().mdft_tune_get(logm, int(blue), int(cols), ctypes.byref(v)))
+    return v.value
+
+
+def tune_set(logm, blue, cols, variant):
+    _check(lib().mdft_tune_set(logm, int(blue), int(cols), int(variant)))
 
 
 class Plan:

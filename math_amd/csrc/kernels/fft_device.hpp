@@ -47,6 +47,16 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b)
     return make_float2(__builtin_fmaf(a.x, b.x, a.y * b.y), __builtin_fmaf(a.y, b.x, -(a.x * b.y)));
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// conj(a * b)
+__device__ __forceinline__ float2 cmul_conj(float2 a, float2 b)
+{
+    return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(-a.x, b.y, -(a.y * b.x)));
+}
+// flip the sign of the imaginary part when sgn is 0x80000000 (conjugate), keep it when 0
+__device__ __forceinline__ float2 cxor(float2 a, uint32_t sgn)
+{
+    return make_float2(a.x, __uint_as_float(__float_as_uint(a.y) ^ sgn));
+}
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
 // ---------------------------------------------------------------------------
@@ -257,9 +267,12 @@ struct FftPass {
     {
         if constexpr (P > 0) {
             __syncthreads();
+            // when TPL is a multiple of 16, pad_pos(t + i*TPL) is pad_pos(t)
+            // plus a compile-time offset (an LDS immediate)
+            const int q0 = pad_pos(t);
 #pragma unroll
             for (int i = 0; i < C::EPT; ++i) {
-                const int q = pad_pos(t + i * C::TPL);
+                const int q = (C::TPL % 16 == 0) ? q0 + i * (C::TPL + C::TPL / 16) : pad_pos(t + i * C::TPL);
                 v[i] = make_float2(re[q], im[q]);
             }
         }
@@ -291,9 +304,15 @@ struct FftPass {
                 const int j = t + b * C::TPL;
                 const int k = j & (NS - 1);
                 const int base = (j - k) * R + k;
+                // NS == 1 (first pass, R == 16): base is a multiple of 16, so
+                // the 16 outputs are contiguous after padding; NS >= 16: NS is
+                // a multiple of 16, so each step adds NS + NS/16
+                static_assert(NS == 1 ? R == 16 : NS % 16 == 0, "linear write offsets");
+                constexpr int step = (NS == 1) ? 1 : NS + NS / 16;
+                const int q0 = pad_pos(base);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int q = pad_pos(base + r * NS);
+                    const int q = q0 + r * step;
                     re[q] = v[b + NB * r].x;
                     im[q] = v[b + NB * r].y;
                 }
@@ -327,54 +346,60 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
     const int line = tile * C::LPW + ll;
     const bool valid = line < job.count;
     const int n = job.n;
-    const int64_t es = job.es;
-    const bool inv = job.inverse != 0;
+    // rows are contiguous (stride 1, known at compile time); a column steps by
+    // the image width.  Consecutive slots of a thread are TPL elements apart.
+    const int64_t es = COLS ? job.es : 1;
+    const int64_t step = int64_t(C::TPL) * es;
+    // direction enters as a sign mask: conjugation is one xor, and the
+    // inverse's 1/n is a multiply by job.scale (exactly 1 for the forward)
+    const uint32_t sgn = job.inverse ? 0x80000000u : 0u;
+    // a power-of-two line fills the FFT (n == M); only Bluestein pads
+    auto in_line = [&](int pos) { return !BLUE || pos < n; };
 
     float2 v[C::EPT];
-    const float2* s = job.src + int64_t(valid ? line : 0) * job.ls;
+    {
+        const float2* s = job.src + int64_t(valid ? line : 0) * job.ls + int64_t(t) * es;
 #pragma unroll
-    for (int i = 0; i < C::EPT; ++i) {
-        const int pos = t + i * C::TPL;
-        float2 x = make_float2(0.0f, 0.0f);
-        if (valid && pos < n) {
-            x = s[pos * es];
-            if constexpr (BLUE) {
-                const float2 c = job.chirp[pos];
-                x = inv ? cmulc(x, c) : cmul(x, c);
-            } else {
-                if (inv) x = cconj(x);
+        for (int i = 0; i < C::EPT; ++i) {
+            const uint32_t pos = uint32_t(t + i * C::TPL);
+            float2 x = make_float2(0.0f, 0.0f);
+            if (valid && in_line((int)pos)) {
+                x = s[i * step];
+                // forward x*c, inverse x*conj(c) (c = chirp); power of two: conj for the inverse
+                x = BLUE ? cmul(x, cxor(job.chirp[pos], sgn)) : cxor(x, sgn);
             }
+            v[i] = x;
         }
-        v[i] = x;
     }
 
     FftTile<LOGM, THREADS, COLS>::run(v, t, ll, lds, job.tw);
 
     if constexpr (BLUE) {
+        // forward conj(v*D), inverse conj(v)*D = conj(v*conj(D)); the outer
+        // conjugation turns the next forward FFT into the inverse one
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
-            const int pos = t + i * C::TPL;
-            const float2 d = job.dtab[pos];
-            v[i] = inv ? cmul(cconj(v[i]), d) : cconj(cmul(v[i], d));
+            const uint32_t pos = uint32_t(t + i * C::TPL);
+            v[i] = cmul_conj(v[i], cxor(job.dtab[pos], sgn));
         }
         FftTile<LOGM, THREADS, COLS>::run(v, t, ll, lds, job.tw);
     }
 
     if (valid) {
-        float2* o = job.dst + int64_t(line) * job.ls;
+        float2* o = job.dst + int64_t(line) * job.ls + int64_t(t) * es;
         const float sc = job.scale;
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
-            const int pos = t + i * C::TPL;
-            if (pos < n) {
-                float2 y = v[i];
+            const uint32_t pos = uint32_t(t + i * C::TPL);
+            if (in_line((int)pos)) {
+                float2 y;
                 if constexpr (BLUE) {
-                    const float2 c = job.chirp[pos];
-                    y = inv ? cscale(cconj(cmul(y, c)), sc) : cmul(cconj(y), c);
+                    // forward conj(v)*c = conj(v*conj(c)), inverse conj(v*c)/n
+                    y = cscale(cmul_conj(v[i], cxor(job.chirp[pos], sgn ^ 0x80000000u)), sc);
                 } else {
-                    if (inv) y = cscale(cconj(y), sc);
+                    y = cscale(cxor(v[i], sgn), sc);
                 }
-                o[pos * es] = y;
+                o[i * step] = y;
             }
         }
     }

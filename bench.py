@@ -51,6 +51,31 @@ def mixed_batch_sizes(count=256, lo=17, hi=2048, p_pow2=0.4, seed=SEED_BASE + 3)
     return sizes, invs
 
 
+def serving_requests(count=20000, lo=32, hi=4096, seed=SEED_BASE + 4):
+    """Request sizes for the serving stream: each side log-uniform in [lo, hi],
+    direction 50/50."""
+    rng = np.random.default_rng(seed)
+    sides = np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi + 1), (count, 2)))).astype(np.int64)
+    sides = np.clip(sides, lo, hi)
+    invs = rng.random(count) < 0.5
+    return [(int(a), int(b)) for a, b in sides], [bool(v) for v in invs], rng
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "batch_mixed256_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per step of the batch kernels, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-scaled), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("TRAFFIC_BYTES_PER_STEP"):
+                return int(line.split()[1])
+    except OSError:
+        pass
+    return None
+
+
 def flops_of(w, h):
     n = w * h
     return 5.0 * n * math.log2(n) if n > 1 else 0.0
@@ -61,10 +86,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--mode", choices=["graph", "direct", "two-phase"], default="graph")
+    ap.add_argument("--mode", choices=["graph", "direct", "persistent", "dependent"], default="graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-extras", action="store_true", help="skip plain-path and accuracy legs")
+    ap.add_argument("--requests", type=int, default=20000, help="serving stream length (0: skip serving)")
+    ap.add_argument("--serve-budget-gib", type=float, default=8.0)
+    ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (scale for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -91,7 +121,8 @@ def main():
     src = torch.randn(total, dtype=torch.complex64, device=dev, generator=g)
     dst = torch.empty_like(src)
 
-    flags = {"graph": M.MDFT_BATCH_GRAPH, "direct": 0, "two-phase": M.MDFT_BATCH_TWO_PHASE}[args.mode]
+    flags = {"graph": M.MDFT_BATCH_GRAPH, "direct": 0, "persistent": M.MDFT_BATCH_PERSISTENT,
+             "dependent": M.MDFT_BATCH_DEPENDENT}[args.mode]
     batch = M.Batch(sizes, invs, offs, offs, flags)
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -133,7 +164,7 @@ def main():
         "data": "synthetic (torch.randn complex64 on device, seeded)",
         "config": {
             "workload": "mixed-size batch: 256 2-D complex<float> DFTs, sides log-uniform in [17,2048] "
-                        "(40% powers of two), N(0,1), mixed forward/inverse, one persistent launch",
+                        "(40% powers of two), N(0,1), mixed forward/inverse, one hipGraph launch per step",
             "transforms": len(sizes),
             "points": total,
             "seed": SEED_BASE + 3,
@@ -148,8 +179,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(gbs_kernel / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "k_batch (persistent, all 256 transforms)",
+            "traffic": measured_traffic(),
+            "traffic_source": "profiles/batch_mixed256_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE per step, "
+                              "scaled by a 1 GiB copy)",
+            "kernel": "k_batch_grid<256> (whole small images + row tiles) then k_batch_grid<1024> (column tiles)",
             "alg_bytes_per_launch": alg_bytes,
             "launch_ms": round(kernel_ms, 4),
         },
@@ -161,6 +194,19 @@ def main():
         out["plain_path"]["speedup_batched_vs_device_plain"] = round(
             value / out["plain_path"]["device_gflops"], 2)
         out["plain_path"]["speedup_batched_vs_host_plain"] = round(value / out["plain_path"]["host_gflops"], 2)
+
+    if args.calibrate_copy:
+        a = torch.empty(1 << 27, dtype=torch.complex64, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = {"read": a.numel() * 8, "write": b.numel() * 8}
+        del a, b
+
+    if args.requests > 0:
+        out["serving"] = serving(M, torch, args.requests, args.serve_budget_gib, args.serve_load)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sizes, invs, args.cpu_seconds)
@@ -219,25 +265,79 @@ def plain_paths(M, torch, src, dst, sizes, invs, offs, flops):
             "host_gflops": round(flops / t_host / 1e9, 1), "host_ms": round(t_host * 1e3, 3)}
 
 
+def serving(M, torch, count, budget_gib, load):
+    """Poisson stream through the continuous-batching server: inputs resident
+    in HBM (a pool of U[-1,1) values, each request reading its own window),
+    admission by a fixed HBM budget, line plans prepared before traffic.  The
+    arrival rate is `load` times the saturated rate measured closed-loop on
+    the first requests of the same stream."""
+    sizes, invs, rng = serving_requests(count)
+    pool_pts = 1 << 28
+    g = torch.Generator(device="cuda")
+    g.manual_seed(SEED_BASE + 4)
+    pool = (torch.rand(pool_pts, dtype=torch.complex64, device="cuda", generator=g) * 2 - (1 + 1j)).contiguous()
+    torch.cuda.synchronize()
+    base = pool.data_ptr()
+    offs = [int(rng.integers(0, pool_pts - w * h + 1)) for (w, h) in sizes]
+    srcs = [base + 8 * o for o in offs]
+    server = M.Server(int(budget_gib * (1 << 30)), streams=4, max_batch=128)
+    t0 = time.perf_counter()
+    server.prepare(sorted({d for wh in sizes for d in wh}))
+    prep_s = time.perf_counter() - t0
+    ws = [w for w, _ in sizes]
+    hs = [h for _, h in sizes]
+    ncal = min(2000, count)
+    _, mk_cal = server.replay(ws[:ncal], hs[:ncal], invs[:ncal], srcs[:ncal], np.zeros(ncal))
+    sat_rate = ncal / (mk_cal * 1e-6)
+    rate = load * sat_rate
+    arrivals = np.cumsum(rng.exponential(1e6 / rate, count))
+    lat, mk = server.replay(ws, hs, invs, srcs, arrivals)
+    st = server.stats()
+    server.close()
+    del pool
+    fl = sum(flops_of(w, h) for (w, h) in sizes)
+    by = sum(32.0 * w * h for (w, h) in sizes)
+    return {
+        "requests": count,
+        "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 4),
+        "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 4),
+        "mean_ms": round(float(lat.mean()) / 1e3, 4),
+        "throughput_req_s": round(count / (mk * 1e-6), 1),
+        "gflops": round(fl / (mk * 1e-6) / 1e9, 1),
+        "gbs": round(by / (mk * 1e-6) / 1e9, 1),
+        "saturated_req_s": round(sat_rate, 1),
+        "arrival_rate_req_s": round(rate, 1),
+        "load": load,
+        "budget_gib": budget_gib,
+        "micro_batches": st["launches"],
+        "mean_batch": round(st["launched_requests"] / max(st["launches"], 1), 2),
+        "plan_prepare_s": round(prep_s, 3),
+        "workload": f"{count} requests, sides log-uniform in [32,4096], U[-1,1), 50/50 forward/inverse, "
+                    "Poisson arrivals, inputs resident in HBM",
+    }
+
+
 def cpu_baseline(sizes, invs, seconds):
     """FP32 restatement of the reference pipeline (oracle, threaded over lines)
-    on the first transforms of the same workload until `seconds` of CPU work."""
+    over the same 256 transforms, repeated until `seconds` of CPU work."""
     from oracle import oracle as O
 
     threads = min(16, os.cpu_count() or 1)
     rng = np.random.default_rng(SEED_BASE + 3)
-    done_flops, done, t_total = 0.0, 0, 0.0
-    for i, (w, h) in enumerate(sizes):
-        x = (rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))).astype(np.complex64)
-        t0 = time.perf_counter()
-        O.dft2d_ref32(x, invs[i], nthreads=threads)
-        t_total += time.perf_counter() - t0
-        done_flops += flops_of(w, h)
-        done += 1
-        if t_total >= seconds:
-            break
+    xs = [(rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))).astype(np.complex64) for (w, h) in sizes]
+    done_flops, done, t_total, passes = 0.0, 0, 0.0, 0
+    while t_total < seconds:
+        for i, (w, h) in enumerate(sizes):
+            t0 = time.perf_counter()
+            O.dft2d_ref32(xs[i], invs[i], nthreads=threads)
+            t_total += time.perf_counter() - t0
+            done_flops += flops_of(w, h)
+            done += 1
+            if t_total >= seconds:
+                break
+        passes += 1
     return {"value": round(done_flops / t_total / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} of {len(sizes)} transforms of the same mixed batch, "
+            "sample": f"{done} transforms ({passes} pass(es) over the same 256-transform mixed batch), "
                       f"{t_total:.1f} s; FP32 restatement of the reference Bluestein/radix-2 pipeline "
                       f"(FFTW absent), threaded over lines"}
 

@@ -132,6 +132,72 @@ int mdft_batch_check(mdft_batch batch);
 
 int mdft_batch_destroy(mdft_batch batch);
 
+/* ---- serving: continuous batching under an HBM budget --------------------- */
+/* No counterpart in the reference (its exec blocks per call,
+ * src/gpu/dft/compute.cpp:257-283).  The server owns `hbm_budget_bytes` of
+ * device memory; each request needs a result slot of width*height*8 bytes
+ * and is admitted in arrival order when its slot fits.  A scheduler thread
+ * launches the admitted requests as micro-batches on `streams` streams (at
+ * most `max_batch` requests each), so requests arriving while others run
+ * join the next launch and overlap with them.  submit/wait/release/stats are
+ * thread-safe; destroy must not race with them. */
+
+typedef struct mdft_server_s* mdft_server;
+
+typedef struct {
+    uint64_t completed;
+    uint64_t pending;           /* submitted, waiting for admission */
+    uint64_t in_flight;         /* admitted and not yet released */
+    uint64_t launches;          /* micro-batches launched */
+    uint64_t launched_requests; /* requests over all micro-batches */
+    uint64_t budget_bytes;
+    uint64_t used_bytes;        /* slots currently held */
+} mdft_server_counters;
+
+typedef struct {
+    uint32_t width;
+    uint32_t height;
+    int32_t inverse;
+    uint32_t reserved;
+    const void* src;   /* device input, read when the request runs */
+    double arrival_us; /* arrival time from the start of the replay */
+} mdft_request;
+
+/* streams = 0 and max_batch = 0 select the defaults (4 and 128). */
+int mdft_server_create(mdft_server* server, uint64_t hbm_budget_bytes, uint32_t streams, uint32_t max_batch);
+
+/* Build the device-resident line plans of these lengths ahead of traffic. */
+int mdft_server_prepare(mdft_server server, const uint32_t* lengths, uint32_t count);
+
+/* Queue a request.  src is a device pointer to width*height complex<float>
+ * that must stay valid until the request completes.  MDFT_ERR_BUDGET when
+ * the slot is larger than the whole budget; MDFT_ERR_SIZE beyond the batch
+ * padded-length limit (8192). */
+int mdft_server_submit(mdft_server server, uint32_t width, uint32_t height, int inverse, const void* src,
+                       uint64_t* ticket);
+
+/* Wait for a request (timeout_us < 0: no limit; MDFT_ERR_TIMEOUT when it
+ * expires).  result receives the device address of the output (valid until
+ * release), latency_us the time from submit to the completion being seen. */
+int mdft_server_wait(mdft_server server, uint64_t ticket, int64_t timeout_us, void** result, double* latency_us);
+
+/* Copy a finished request's result (count complex<float>) to host memory. */
+int mdft_server_fetch(mdft_server server, uint64_t ticket, void* host_dst, size_t count);
+
+/* Return a finished request's slot to the budget. */
+int mdft_server_release(mdft_server server, uint64_t ticket);
+
+int mdft_server_stats(mdft_server server, mdft_server_counters* stats);
+
+/* Replay a request trace: submits each request at its arrival time from one
+ * thread while another waits for and releases them in order, recording each
+ * latency (latency_us[i]) and the time from the first arrival to the last
+ * completion. */
+int mdft_server_replay(mdft_server server, const mdft_request* requests, uint32_t count, double* latency_us,
+                       double* makespan_us);
+
+int mdft_server_destroy(mdft_server server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,8 @@ SOURCES = [
     "host/runtime.cpp",
     "host/capi.cpp",
     "host/batch.cpp",
+    "host/batch_builder.cpp",
+    "host/server.cpp",
 ]
 
 

@@ -1,0 +1,152 @@
+// Job tables and work lists of batched transforms.
+//
+// Each image contributes a row job and a column job, cut into tiles of whole
+// lines.  A small image whose rows and columns each fit one tile becomes a
+// single "whole" item done by one workgroup.  Without dependencies, items
+// are sorted by phase (whole images and row tiles first, column tiles
+// second) and then by kernel variant (padded length, Bluestein or not, axis)
+// so neighbouring workgroups run the same code.  A line length gets the same
+// radix schedule in every batch, so a result is bitwise the same alone or in
+// any mix.
+#include "batch_builder.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <string>
+#include <tuple>
+
+#include "../../../include/mdft.h"
+
+namespace mdft {
+
+int batch_threads_for(const uint32_t* widths, const uint32_t* heights, size_t count)
+{
+    const int limit = batch_max_logm(1024);
+    int maxlogm = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const int l1 = ilog2(compute_m(widths[i])), l2 = ilog2(compute_m(heights[i]));
+        if (l1 > limit || l2 > limit)
+            throw Error(MDFT_ERR_SIZE, "batch transform " + std::to_string(i) + " (" + std::to_string(widths[i]) + "x" +
+                                           std::to_string(heights[i]) + ") exceeds the batch padded-length limit " +
+                                           std::to_string(1 << limit));
+        maxlogm = std::max({maxlogm, l1, l2});
+    }
+    return maxlogm > batch_max_logm(256) ? 1024 : 256;
+}
+
+void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, int TC, bool dependent,
+                 BatchBuild& out, std::vector<uint32_t>* job_image, std::vector<uint8_t>* job_is_col)
+{
+    out = BatchBuild{};
+    out.threads = T;
+    out.threads_cols = TC;
+    if (job_image) job_image->clear();
+    if (job_is_col) job_is_col->clear();
+
+    std::vector<uint32_t> order(count);
+    std::iota(order.begin(), order.end(), 0u);
+    std::vector<double> work(count);
+    for (size_t i = 0; i < count; ++i)
+        work[i] = (double)compute_m(entries[i].width) * entries[i].height +
+                  (double)compute_m(entries[i].height) * entries[i].width;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return work[a] > work[c]; });
+
+    struct Item {
+        int phase;
+        int logm, blue, kind;  // kind: 0 whole, 1 row tile, 2 column tile
+        uint32_t seq;
+        uint2 v;
+    };
+    std::vector<Item> items;
+    std::vector<std::pair<std::vector<uint2>, std::vector<uint2>>> per_image;  // dependency ordering
+    uint32_t counter = 0, seq = 0;
+    for (uint32_t idx : order) {
+        const BatchEntry& e = entries[idx];
+        const int w = (int)e.width, h = (int)e.height;
+        auto rp = ctx.line_plan(w);
+        auto cp = ctx.line_plan(h);
+        out.keep.push_back(rp);
+        out.keep.push_back(cp);
+        const int lpw_r = w > 1 ? batch_lines_per_tile(T, rp->logm) : 0;
+        // a whole image runs both passes in one row-phase workgroup
+        const int lpw_cw = h > 1 ? batch_lines_per_tile(T, cp->logm) : 0;
+        const bool whole = (w > 1 && h > 1 && h <= lpw_r && w <= lpw_cw);
+        // column tiles of images with a row pass run in the column phase (TC
+        // threads); a single-column-pass image (w == 1) runs with the rows
+        const int lpw_c = h > 1 ? batch_lines_per_tile((w > 1 && !whole) ? TC : T, cp->logm) : 0;
+        const int ntr = w > 1 ? (h + lpw_r - 1) / lpw_r : 0;
+        const int ntc = h > 1 ? (w + lpw_c - 1) / lpw_c : 0;
+        const int sig = (dependent && w > 1 && h > 1 && !whole) ? (int)counter++ : -1;
+        int rj = -1, cj = -1;
+        if (w > 1) {
+            BatchJob j{};
+            j.job = ctx.make_job(*rp, e.src, e.dst, 1, w, h, e.inverse);
+            j.logm = rp->logm;
+            j.blue = rp->blue;
+            j.cols = 0;
+            j.ntiles = ntr;
+            j.dep = -1;
+            j.sig = sig;
+            rj = (int)out.jobs.size();
+            out.jobs.push_back(j);
+            if (job_image) job_image->push_back(idx);
+            if (job_is_col) job_is_col->push_back(0);
+        }
+        if (h > 1) {
+            BatchJob j{};
+            // the column pass reads the row pass's output when there is one
+            j.job = ctx.make_job(*cp, w > 1 ? e.dst : e.src, e.dst, w, 1, w, e.inverse);
+            j.logm = cp->logm;
+            j.blue = cp->blue;
+            j.cols = 1;
+            j.ntiles = ntc;
+            j.dep = sig;
+            j.dep_need = ntr;
+            j.sig = -1;
+            cj = (int)out.jobs.size();
+            out.jobs.push_back(j);
+            if (job_image) job_image->push_back(idx);
+            if (job_is_col) job_is_col->push_back(1);
+        }
+        std::vector<uint2> ri, ci;
+        if (whole) {
+            items.push_back({1, rp->logm, rp->blue, 0, seq++, make_uint2(rj, BATCH_WHOLE | (uint32_t)cj)});
+            ri.push_back(items.back().v);
+        } else {
+            for (int t = 0; t < ntr; ++t) {
+                items.push_back({1, rp->logm, rp->blue, 1, seq++, make_uint2(rj, t)});
+                ri.push_back(items.back().v);
+            }
+            for (int t = 0; t < ntc; ++t) {
+                items.push_back({w > 1 ? 2 : 1, cp->logm, cp->blue, 2, seq++, make_uint2(cj, t)});
+                ci.push_back(items.back().v);
+            }
+        }
+        per_image.push_back({ri, ci});
+    }
+
+    if (dependent) {
+        // rows of image k+1 before columns of image k: a column tile only
+        // waits on row tiles with smaller tickets
+        for (size_t k = 0; k < per_image.size(); ++k) {
+            out.items.insert(out.items.end(), per_image[k].first.begin(), per_image[k].first.end());
+            if (k > 0)
+                out.items.insert(out.items.end(), per_image[k - 1].second.begin(), per_image[k - 1].second.end());
+        }
+        if (!per_image.empty())
+            out.items.insert(out.items.end(), per_image.back().second.begin(), per_image.back().second.end());
+        out.nphase1 = (uint32_t)out.items.size();
+    } else {
+        std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) {
+            return std::make_tuple(a.phase, -a.logm, a.blue, a.kind, a.seq) <
+                   std::make_tuple(c.phase, -c.logm, c.blue, c.kind, c.seq);
+        });
+        for (const Item& it : items) {
+            if (it.phase == 1) ++out.nphase1;
+            out.items.push_back(it.v);
+        }
+    }
+    out.ncounters = counter;
+}
+
+}  // namespace mdft

@@ -7,8 +7,10 @@
 // synchronisation at all.  Three launch shapes share the item code:
 //
 //  * k_batch_grid: one workgroup per item (the hardware dispatcher balances
-//    the load); a batch is two launches, whole images + row tiles, then
-//    column tiles.
+//    the load); a batch is two launches, whole images + row tiles (256-thread
+//    tiles), then column tiles (1024-thread tiles, 4-16 columns each for
+//    padded lengths 1024-4096, so each row of a tile is a longer contiguous
+//    run).
 //  * k_batch_persistent: a grid sized to the device's residency takes items
 //    from a ticket counter, one launch per phase.
 //  * k_batch_persistent with dependencies: one launch for everything; a
@@ -28,6 +30,18 @@
 namespace mdft {
 
 __host__ __device__ constexpr int batch_maxlogm(int threads) { return threads >= 512 ? 13 : 12; }
+
+// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Within each run of 64 blocks, give XCD x the 8 consecutive items
+// 8x..8x+7, so neighbouring tiles of one job (columns sharing cache lines)
+// run at the same time in the same L2.  A partial last run keeps its order.
+__device__ __forceinline__ uint32_t xcd_group_perm(uint32_t p, uint32_t n)
+{
+    const uint32_t g = p >> 6;
+    if ((g + 1) * 64 > n) return p;
+    const uint32_t q = p & 63;
+    return (g << 6) + ((q & 7) << 3) + (q >> 3);
+}
 
 template <int LOGM, int T>
 constexpr int lds_need()
@@ -137,13 +151,14 @@ __device__ __forceinline__ void run_item(const BatchJob* __restrict__ jobs, uint
     }
 }
 
-// 256-thread batches run four waves per SIMD (<= 128 VGPRs), 512-thread
-// batches two.
+// Four waves per SIMD (<= 128 VGPRs): 256-thread tiles (4 workgroups per
+// CU) or 1024-thread tiles (one, with 4x the lines per tile).
 template <int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T == 256 ? 4 : 2))) void k_batch_grid(const BatchJob* __restrict__ jobs, const uint2* __restrict__ items)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_grid(
+    const BatchJob* __restrict__ jobs, const uint2* __restrict__ items)
 {
     __shared__ float lds[batch_lds<T>()];
-    run_item<T, true>(jobs, items[blockIdx.x], lds);
+    run_item<T, true>(jobs, items[xcd_group_perm(blockIdx.x, gridDim.x)], lds);
 }
 
 // The item loop contains barriers, so its control flow is kept free of
@@ -155,9 +170,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T == 256 ? 4 
 // Deadlock freedom of the dependency mode: an item waits only on items with
 // smaller tickets, taken by running workgroups that never wait on later ones.
 template <int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T == 256 ? 4 : 2))) void k_batch_persistent(const BatchJob* __restrict__ jobs,
-                                                        const uint2* __restrict__ items, uint32_t nitems,
-                                                        uint32_t* ctl, uint32_t spin_limit)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_persistent(
+    const BatchJob* __restrict__ jobs, const uint2* __restrict__ items, uint32_t nitems, uint32_t* ctl,
+    uint32_t spin_limit)
 {
     __shared__ float lds[batch_lds<T>()];
     __shared__ uint32_t s_item;
@@ -214,7 +229,7 @@ hipError_t launch_batch_grid(int threads, const BatchJob* jobs, const uint2* ite
     if (threads == 256)
         hipLaunchKernelGGL(k_batch_grid<256>, dim3(nitems), dim3(256), 0, stream, jobs, items);
     else
-        hipLaunchKernelGGL(k_batch_grid<512>, dim3(nitems), dim3(512), 0, stream, jobs, items);
+        hipLaunchKernelGGL(k_batch_grid<1024>, dim3(nitems), dim3(1024), 0, stream, jobs, items);
     return hipGetLastError();
 }
 
@@ -228,7 +243,7 @@ hipError_t launch_batch_persistent(int threads, const BatchJob* jobs, const uint
         hipLaunchKernelGGL(k_batch_persistent<256>, dim3(grid), dim3(256), 0, stream, jobs, items, nitems, ctl,
                            spin_limit);
     else
-        hipLaunchKernelGGL(k_batch_persistent<512>, dim3(grid), dim3(512), 0, stream, jobs, items, nitems, ctl,
+        hipLaunchKernelGGL(k_batch_persistent<1024>, dim3(grid), dim3(1024), 0, stream, jobs, items, nitems, ctl,
                            spin_limit);
     return hipGetLastError();
 }
@@ -244,7 +259,7 @@ hipError_t batch_occupancy(int threads, int* blocks_per_cu)
 {
     if (threads == 256)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_persistent<256>, 256, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_persistent<512>, 512, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_persistent<1024>, 1024, 0);
 }
 
 int batch_max_logm(int threads) { return batch_maxlogm(threads); }

@@ -32,7 +32,7 @@ hipError_t gen_dtab(float2* dtab, const double2* chirp64, double* cos_scratch, i
 
 hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, bool wide, hipStream_t stream);
 
-// batch kernels: threads is 256 (padded lengths up to 4096) or 512 (up to 8192)
+// batch kernels: threads is 256 (padded lengths up to 4096) or 1024 (up to 8192)
 hipError_t launch_batch_grid(int threads, const BatchJob* jobs, const uint2* items, uint32_t nitems,
                              hipStream_t stream);
 hipError_t launch_batch_persistent(int threads, const BatchJob* jobs, const uint2* items, uint32_t nitems,

@@ -54,6 +54,22 @@ class mdft_desc(ctypes.Structure):
     ]
 
 
+class mdft_server_counters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("completed", "pending", "in_flight", "launches", "launched_requests",
+                                                "budget_bytes", "used_bytes")]
+
+
+class mdft_request(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("inverse", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32),
+        ("src", ctypes.c_void_p),
+        ("arrival_us", ctypes.c_double),
+    ]
+
+
 _vp = ctypes.c_void_p
 _L = None
 
@@ -93,6 +109,19 @@ def lib():
             "mdft_batch_exec_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
             "mdft_batch_check": ([_vp], ctypes.c_int),
             "mdft_batch_destroy": ([_vp], ctypes.c_int),
+            "mdft_server_create": ([ctypes.POINTER(_vp), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32],
+                                   ctypes.c_int),
+            "mdft_server_prepare": ([_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32], ctypes.c_int),
+            "mdft_server_submit": ([_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp,
+                                    ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+            "mdft_server_wait": ([_vp, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_vp),
+                                  ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+            "mdft_server_release": ([_vp, ctypes.c_uint64], ctypes.c_int),
+            "mdft_server_fetch": ([_vp, ctypes.c_uint64, _vp, ctypes.c_size_t], ctypes.c_int),
+            "mdft_server_stats": ([_vp, ctypes.POINTER(mdft_server_counters)], ctypes.c_int),
+            "mdft_server_replay": ([_vp, ctypes.POINTER(mdft_request), ctypes.c_uint32,
+                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+            "mdft_server_destroy": ([_vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -223,6 +252,66 @@ class Batch:
     def close(self):
         if self._h:
             lib().mdft_batch_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server with HBM-budget admission (mdft_server_*)."""
+
+    def __init__(self, budget_bytes, streams=0, max_batch=0):
+        self._h = _vp()
+        _check(lib().mdft_server_create(ctypes.byref(self._h), int(budget_bytes), int(streams), int(max_batch)))
+
+    def prepare(self, lengths):
+        arr = (ctypes.c_uint32 * max(len(lengths), 1))(*[int(n) for n in lengths])
+        _check(lib().mdft_server_prepare(self._h, arr, len(lengths)))
+
+    def submit(self, width, height, inverse, src_ptr):
+        t = ctypes.c_uint64()
+        _check(lib().mdft_server_submit(self._h, int(width), int(height), int(bool(inverse)), src_ptr,
+                                        ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout_us=-1):
+        """Returns (device address of the result, latency in microseconds)."""
+        r, lat = _vp(), ctypes.c_double()
+        _check(lib().mdft_server_wait(self._h, ticket, int(timeout_us), ctypes.byref(r), ctypes.byref(lat)))
+        return r.value, lat.value
+
+    def fetch(self, ticket, count):
+        """Copy a finished result to a new complex64 host array of `count` values."""
+        out = np.empty(count, np.complex64)
+        _check(lib().mdft_server_fetch(self._h, ticket, out.ctypes.data, count))
+        return out
+
+    def release(self, ticket):
+        _check(lib().mdft_server_release(self._h, ticket))
+
+    def stats(self):
+        st = mdft_server_counters()
+        _check(lib().mdft_server_stats(self._h, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in mdft_server_counters._fields_}
+
+    def replay(self, widths, heights, inverses, src_ptrs, arrivals_us):
+        n = len(widths)
+        reqs = (mdft_request * max(n, 1))()
+        for i in range(n):
+            reqs[i] = mdft_request(int(widths[i]), int(heights[i]), int(bool(inverses[i])), 0, int(src_ptrs[i]),
+                                   float(arrivals_us[i]))
+        lat = (ctypes.c_double * max(n, 1))()
+        mk = ctypes.c_double()
+        _check(lib().mdft_server_replay(self._h, reqs, n, lat, ctypes.byref(mk)))
+        return np.frombuffer(lat, dtype=np.float64, count=n).copy(), mk.value
+
+    def close(self):
+        if self._h:
+            lib().mdft_server_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

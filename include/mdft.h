@@ -98,6 +98,69 @@ int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, vo
 int mdft_tune_get(int logm, int blue, int cols, int* variant);
 int mdft_tune_set(int logm, int blue, int cols, int variant);
 
+/* ---- batched power-of-two 1-D FFT ------------------------------------------ */
+
+typedef struct mdft_fft_s* mdft_fft;
+
+/* Plan `count` FFTs of power-of-two length n stored back to back (line j at
+ * elements j*n .. j*n+n-1).  Forward unnormalised, inverse scaled by 1/n
+ * (the reference divides by its tile length at the store and halves at every
+ * global stage, src/gpu/dft/code/dft_fft_shared.comp:72,
+ * dft_fft_global.comp:72-77).  n == 1 is a no-op plan.  Replaces
+ * create_fft(device, family_indices, count, n) (src/gpu/dft/fft.h:49-53,
+ * fft.cpp:293-330).  Errors (MDFT_ERR_SIZE, reference texts of
+ * fft.cpp:298-306): "FFT size <n> is not positive", "FFT size <n> is not an
+ * integral power of 2"; n above 2^28 or count*n above 2^40 are rejected. */
+int mdft_fft_create(mdft_fft* fft, uint32_t count, uint32_t n);
+
+/* In-place FFT of size complex<float> values in host memory, blocking.
+ * Replaces Fft::run_for_data(inverse, data, ...) (fft.h:39-44,
+ * fft.cpp:264-290).  Error MDFT_ERR_DATA_SIZE "FFT buffer size <size> must be
+ * equal to data size <count*n>" (fft.cpp:276-280); a plan of n == 1 returns
+ * without checking, as the reference does. */
+int mdft_fft_exec(mdft_fft fft, int inverse, void* data, size_t size);
+
+/* Enqueue the FFT of device memory on a stream (src may equal dst); the
+ * analogue of Fft::commands(command_buffer, inverse) (fft.h:37,
+ * fft.cpp:243-262).  n == 1 copies src to dst when they differ. */
+int mdft_fft_exec_device(mdft_fft fft, int inverse, const void* src, void* dst, void* stream);
+
+int mdft_fft_destroy(mdft_fft fft);
+
+/* ---- the view's image path ------------------------------------------------- */
+
+#define MDFT_PIXEL_RGBA32F 0u /* 4 x float per pixel (r, g, b, a) */
+#define MDFT_PIXEL_RGBA8 1u   /* 4 x uint8 per pixel, UNORM (value / 255) */
+
+typedef struct mdft_image_s* mdft_image;
+
+/* Plan the image transform of a width x height rectangle: the luminance
+ * 0.2126 r + 0.7152 g + 0.0722 b of each source pixel is the complex input
+ * (src/gpu/dft/code/dft_copy_input.comp:37-54), the forward 2-D DFT runs, and
+ * |X| / (width*height) is written to an R32F image of the rectangle's size
+ * (dft_copy_output.comp:34-48, scale from compute.cpp:128-129).  Replaces
+ * ComputeImage::create_buffers(sampler, input, output, rectangle, family)
+ * (src/gpu/dft/compute.h:35-50, compute.cpp:103-134); the images are bound at
+ * exec time.  Errors as mdft_plan_create, and MDFT_ERR_INVALID for an unknown
+ * pixel format. */
+int mdft_image_create(mdft_image* image, uint32_t width, uint32_t height, uint32_t pixel_format);
+
+/* Enqueue the image transform on a stream.  src: device pixels of a
+ * src_width x src_height image with a row pitch of src_pitch pixels; the
+ * rectangle's origin is (x, y).  dst: device floats with a row pitch of
+ * dst_pitch floats.  The luminance read is fused into the first axis pass and
+ * the magnitude write into the last.  Replaces
+ * ComputeImage::compute_commands(command_buffer) (compute.h:40,
+ * compute.cpp:146-176).  MDFT_ERR_INVALID when the rectangle does not lie
+ * inside the source or a pitch is smaller than its row (the reference asserts
+ * these, compute.cpp:113-118). */
+int mdft_image_exec_device(mdft_image image, const void* src, uint32_t src_width, uint32_t src_height,
+                           uint64_t src_pitch, uint32_t x, uint32_t y, float* dst, uint64_t dst_pitch, void* stream);
+
+/* Free the plan (ComputeImage::delete_buffers, compute.h:49); waits for the
+ * device first.  NULL is accepted. */
+int mdft_image_destroy(mdft_image image);
+
 /* ---- many transforms of mixed sizes in one persistent launch ------------- */
 
 typedef struct {

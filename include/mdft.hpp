@@ -123,6 +123,30 @@ inline std::unique_ptr<DFT> create_hip(int x, int y)
     return std::make_unique<detail::HipDft>(x, y);
 }
 
+// count power-of-two FFTs of length n stored back to back, in place; the shape
+// of ns::gpu::dft::Fft and create_fft (src/gpu/dft/fft.h:32-53) without the
+// Vulkan handles.  Inverse is scaled by 1/n.
+class Fft {
+    mdft_fft f_ = nullptr;
+
+public:
+    Fft(unsigned count, unsigned n) { check(mdft_fft_create(&f_, count, n)); }
+    ~Fft() { mdft_fft_destroy(f_); }
+    Fft(const Fft&) = delete;
+    Fft& operator=(const Fft&) = delete;
+    void run_for_data(bool inverse, std::vector<std::complex<float>>* data)
+    {
+        check(mdft_fft_exec(f_, inverse ? 1 : 0, data ? data->data() : nullptr, data ? data->size() : 0));
+    }
+    // the analogue of Fft::commands(command_buffer, inverse): enqueue on a stream
+    void commands(bool inverse, const void* src, void* dst, void* stream = nullptr) const
+    {
+        check(mdft_fft_exec_device(f_, inverse ? 1 : 0, src, dst, stream));
+    }
+};
+
+inline std::unique_ptr<Fft> create_fft(unsigned count, unsigned n) { return std::make_unique<Fft>(count, n); }
+
 // Many transforms of mixed sizes (mdft_batch_*).
 class Batch {
     mdft_batch b_ = nullptr;

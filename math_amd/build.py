@@ -29,8 +29,12 @@ SOURCES = [
     "kernels/axis.hip",
     "kernels/batch.hip",
     "kernels/tables.hip",
+    "kernels/large.hip",
+    "kernels/image.hip",
     "host/runtime.cpp",
     "host/capi.cpp",
+    "host/fft1d.cpp",
+    "host/image.cpp",
     "host/batch.cpp",
     "host/batch_builder.cpp",
     "host/server.cpp",

@@ -1,6 +1,7 @@
 // C ABI of the single-transform path: plans, host and device exec, errors.
 #include "capi_util.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -13,6 +14,8 @@ struct mdft_plan_s {
     uint32_t w = 0, h = 0;
     std::shared_ptr<const LinePlan> row, col;
     float2* xd = nullptr;          // device image for mdft_exec, allocated on first use
+    float2* scratch = nullptr;     // long-line passes only
+    uint64_t scratch_elems = 0;
     hipStream_t stream = nullptr;  // private stream of mdft_exec
 };
 
@@ -53,6 +56,12 @@ int mdft_plan_create(mdft_plan* plan, uint32_t width, uint32_t height)
     p->h = height;
     p->row = ctx.line_plan((int)width);
     p->col = ctx.line_plan((int)height);
+    p->scratch_elems = std::max(width > 1 ? p->row->scratch_elems((int)height) : uint64_t(0),
+                                height > 1 ? p->col->scratch_elems((int)width) : uint64_t(0));
+    if (p->scratch_elems) {
+        mdft::DeviceGuard g(p->device);
+        mdft::hip_check(hipMalloc(&p->scratch, sizeof(float2) * p->scratch_elems), "hipMalloc scratch");
+    }
     *plan = p.release();
     MDFT_API_END
 }
@@ -68,6 +77,7 @@ int mdft_plan_destroy(mdft_plan plan)
             (void)hipStreamDestroy(plan->stream);
         }
         if (plan->xd) (void)hipFree(plan->xd);
+        if (plan->scratch) (void)hipFree(plan->scratch);
     }
     delete plan;
     MDFT_API_END
@@ -81,7 +91,8 @@ int mdft_plan_info(mdft_plan plan, uint32_t* m1, uint32_t* m2, uint64_t* device_
     if (m2) *m2 = (uint32_t)plan->col->m;
     if (device_bytes)
         *device_bytes = plan->row->bytes + (plan->col != plan->row ? plan->col->bytes : 0) +
-                        (plan->xd ? sizeof(float2) * (uint64_t)plan->w * plan->h : 0);
+                        (plan->xd ? sizeof(float2) * (uint64_t)plan->w * plan->h : 0) +
+                        sizeof(float2) * plan->scratch_elems;
     MDFT_API_END
 }
 
@@ -98,7 +109,7 @@ int mdft_exec(mdft_plan plan, int inverse, void* data, size_t count)
     Context& ctx = Context::get(plan->device);
     mdft::hip_check(hipMemcpyAsync(plan->xd, data, bytes, hipMemcpyHostToDevice, plan->stream), "copy in");
     ctx.enqueue_transform(*plan->row, *plan->col, (int)plan->w, (int)plan->h, inverse != 0, plan->xd, plan->xd,
-                          plan->stream);
+                          plan->scratch, plan->stream);
     mdft::hip_check(hipMemcpyAsync(data, plan->xd, bytes, hipMemcpyDeviceToHost, plan->stream), "copy out");
     mdft::hip_check(hipStreamSynchronize(plan->stream), "exec");
     MDFT_API_END
@@ -112,7 +123,7 @@ int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, vo
     mdft::DeviceGuard g(plan->device);
     Context& ctx = Context::get(plan->device);
     ctx.enqueue_transform(*plan->row, *plan->col, (int)plan->w, (int)plan->h, inverse != 0,
-                          static_cast<const float2*>(src), static_cast<float2*>(dst),
+                          static_cast<const float2*>(src), static_cast<float2*>(dst), plan->scratch,
                           static_cast<hipStream_t>(stream));
     MDFT_API_END
 }

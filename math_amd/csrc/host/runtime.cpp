@@ -39,6 +39,8 @@ LinePlan::~LinePlan()
 {
     if (chirp) (void)hipFree(chirp);
     if (dtab) (void)hipFree(dtab);
+    if (tw_lo) (void)hipFree(tw_lo);
+    if (tw_hi) (void)hipFree(tw_hi);
 }
 
 DeviceGuard::DeviceGuard(int device)
@@ -104,13 +106,24 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
     lp->m = m;
     lp->logm = ilog2((uint64_t)m);
     lp->blue = !is_pow2((uint64_t)n);
-    if (lp->logm > MAX_LOGM)
+    if (lp->logm > LARGE_MAX_LOGM)
         throw Error(MDFT_ERR_SIZE, "Error size " + std::to_string(n) + ": padded length " + std::to_string(m) +
-                                       " exceeds the longest supported line (" + std::to_string(1 << MAX_LOGM) + ")");
+                                       " exceeds the longest supported line (2^" + std::to_string(LARGE_MAX_LOGM) +
+                                       ")");
+    DeviceGuard g(device_);
+    if (lp->large()) {
+        lp->logm1 = lp->logm - lp->logm / 2;
+        lp->logm2 = lp->logm / 2;
+        const int64_t nhi = int64_t(m) >> LARGE_LO_BITS;
+        hip_check(hipMalloc(&lp->tw_lo, sizeof(float2) * LARGE_LO), "hipMalloc twiddles");
+        hip_check(hipMalloc(&lp->tw_hi, sizeof(float2) * nhi), "hipMalloc twiddles");
+        lp->bytes += sizeof(float2) * (uint64_t)(LARGE_LO + nhi);
+        hip_check(gen_large_twiddles(lp->tw_lo, lp->tw_hi, m, gen_stream_), "twiddle generation");
+    }
     if (lp->blue) {
-        DeviceGuard g(device_);
         const size_t need = 2 * (size_t)n + (size_t)m;
         if (need > scratch64_doubles_) {
+            hip_check(hipStreamSynchronize(gen_stream_), "table generation");
             if (scratch64_) hip_check(hipFree(scratch64_), "hipFree");
             scratch64_ = nullptr;
             hip_check(hipMalloc(&scratch64_, sizeof(double) * need), "hipMalloc scratch");
@@ -118,12 +131,35 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
         }
         hip_check(hipMalloc(&lp->chirp, sizeof(float2) * (size_t)n), "hipMalloc chirp");
         hip_check(hipMalloc(&lp->dtab, sizeof(float2) * (size_t)m), "hipMalloc dtab");
-        lp->bytes = sizeof(float2) * ((size_t)n + (size_t)m);
+        lp->bytes += sizeof(float2) * ((size_t)n + (size_t)m);
         double2* chirp64 = reinterpret_cast<double2*>(scratch64_);
         hip_check(gen_chirp(chirp64, lp->chirp, n, gen_stream_), "chirp generation");
-        hip_check(gen_dtab(lp->dtab, chirp64, scratch64_ + 2 * (size_t)n, n, m, gen_stream_), "diagonal generation");
-        hip_check(hipStreamSynchronize(gen_stream_), "table generation");
+        if (!lp->large()) {
+            // D = FFT_M(h2)/M by a direct FP64 sum with exact index reduction
+            hip_check(gen_dtab(lp->dtab, chirp64, scratch64_ + 2 * (size_t)n, n, m, gen_stream_),
+                      "diagonal generation");
+        } else {
+            // long lines: D = FFT_M(h2)/M by this library's four-step FFT on the
+            // FP32-rounded embedding (the reference computes its D with an FP32 FFT
+            // as well, dft.cpp:57-69)
+            float2* tmp = nullptr;
+            hip_check(hipMalloc(&tmp, sizeof(float2) * 2 * (size_t)m), "hipMalloc diagonal scratch");
+            hip_check(gen_h2_f32(tmp, chirp64, n, m, gen_stream_), "embedding");
+            LinePlan pw;  // the power-of-two FFT of length m, sharing this plan's twiddles
+            pw.n = m;
+            pw.m = m;
+            pw.logm = lp->logm;
+            pw.logm1 = lp->logm1;
+            pw.logm2 = lp->logm2;
+            pw.tw_lo = lp->tw_lo;
+            pw.tw_hi = lp->tw_hi;
+            enqueue_large_fft(pw, tmp, lp->dtab, 1, m, 1, 1.0f / (float)m, tmp + m, gen_stream_);
+            pw.tw_lo = pw.tw_hi = nullptr;  // owned by lp
+            hip_check(hipStreamSynchronize(gen_stream_), "diagonal generation");
+            hip_check(hipFree(tmp), "hipFree");
+        }
     }
+    hip_check(hipStreamSynchronize(gen_stream_), "table generation");
     lines_[n] = lp;
     return lp;
 }
@@ -154,24 +190,104 @@ AxisJob Context::make_job(const LinePlan& lp, const float2* src, float2* dst, in
     return j;
 }
 
-void Context::enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
-                           bool inverse, bool cols, hipStream_t stream) const
+// Step of a long-line FFT: sub-FFTs of length 2^sublogm.
+static void large_step(int sublogm, const float2* tw, LargeJob job, hipStream_t stream)
 {
+    job.tw = tw;
+    hip_check(launch_large_step(sublogm, job, stream), "long-line step launch");
+}
+
+void Context::enqueue_large_fft(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
+                                int64_t count, float scale, float2* scratch, hipStream_t stream) const
+{
+    // forward FFT (power of two, n == m) of count lines: A then C
+    const int64_t M = lp.m, M1 = int64_t(1) << lp.logm1, M2 = int64_t(1) << lp.logm2;
+    LargeJob a{};
+    a.src = src;
+    a.dst = scratch;
+    a.sub_in = M1, a.ls_in = ls, a.ls2_in = es, a.es_in = M1 * es;
+    a.sub_out = M1, a.ls_out = M, a.ls2_out = M2, a.es_out = 1;
+    a.count = count * M1;
+    a.load_mode = LARGE_LOAD_PLAIN, a.store_mode = LARGE_STORE_TWIDDLE;
+    a.n = M, a.m1 = M1, a.m2 = M2, a.scale = 1.0f;
+    a.tw_lo = lp.tw_lo, a.tw_hi = lp.tw_hi;
+    large_step(lp.logm2, twiddles(lp.logm2), a, stream);
+    LargeJob c{};
+    c.src = scratch;
+    c.dst = dst;
+    c.sub_in = M2, c.ls_in = M, c.ls2_in = 1, c.es_in = M2;
+    c.sub_out = M2, c.ls_out = ls, c.ls2_out = es, c.es_out = M2 * es;
+    c.count = count * M2;
+    c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = LARGE_STORE_CONJ;  // sign 0: a plain store times scale
+    c.n = M, c.m1 = M1, c.m2 = M2, c.scale = scale;
+    large_step(lp.logm1, twiddles(lp.logm1), c, stream);
+}
+
+void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
+                                 int64_t count, bool inverse, float2* scratch, hipStream_t stream) const
+{
+    const int64_t M = lp.m, M1 = int64_t(1) << lp.logm1, M2 = int64_t(1) << lp.logm2;
+    const float scale = inverse ? 1.0f / (float)lp.n : 1.0f;
+    float2* S = scratch;                      // step A output, count * M
+    float2* P = scratch + (size_t)count * M;  // Bluestein padded buffer, count * M
+    auto step_a = [&](const float2* from, int64_t fls, int64_t fes, int load_mode) {
+        LargeJob a{};
+        a.src = from;
+        a.dst = S;
+        a.sub_in = M1, a.ls_in = fls, a.ls2_in = fes, a.es_in = M1 * fes;
+        a.sub_out = M1, a.ls_out = M, a.ls2_out = M2, a.es_out = 1;
+        a.count = count * M1;
+        a.load_mode = load_mode, a.store_mode = LARGE_STORE_TWIDDLE, a.inverse = inverse ? 1 : 0;
+        a.n = lp.n, a.m1 = M1, a.m2 = M2, a.scale = 1.0f;
+        a.chirp = lp.chirp, a.tw_lo = lp.tw_lo, a.tw_hi = lp.tw_hi;
+        large_step(lp.logm2, twiddles(lp.logm2), a, stream);
+    };
+    auto step_c = [&](float2* to, int64_t tls, int64_t tes, int store_mode, float sc) {
+        LargeJob c{};
+        c.src = S;
+        c.dst = to;
+        c.sub_in = M2, c.ls_in = M, c.ls2_in = 1, c.es_in = M2;
+        c.sub_out = M2, c.ls_out = tls, c.ls2_out = tes, c.es_out = M2 * tes;
+        c.count = count * M2;
+        c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = store_mode, c.inverse = inverse ? 1 : 0;
+        c.n = lp.n, c.m1 = M1, c.m2 = M2, c.scale = sc;
+        c.chirp = lp.chirp, c.dtab = lp.dtab;
+        large_step(lp.logm1, twiddles(lp.logm1), c, stream);
+    };
+    if (!lp.blue) {
+        step_a(src, ls, es, LARGE_LOAD_CONJ);         // conj(x) for the inverse
+        step_c(dst, ls, es, LARGE_STORE_CONJ, scale);  // conj(v)/n for the inverse
+        return;
+    }
+    step_a(src, ls, es, LARGE_LOAD_CHIRP);    // x * c, zero past n
+    step_c(P, M, 1, LARGE_STORE_DIAG, 1.0f);  // conj(v * D), natural order, padded
+    step_a(P, M, 1, LARGE_LOAD_PLAIN);
+    step_c(dst, ls, es, LARGE_STORE_CHIRP, scale);  // conj(v) * c, crop to n, 1/n for the inverse
+}
+
+void Context::enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
+                           bool inverse, bool cols, float2* scratch, hipStream_t stream) const
+{
+    if (lp.large()) {
+        if (!scratch) throw Error(MDFT_ERR_STATE, "long-line pass without a scratch buffer");
+        enqueue_large_axis(lp, src, dst, es, ls, count, inverse, scratch, stream);
+        return;
+    }
     const AxisJob j = make_job(lp, src, dst, es, ls, count, inverse);
     hip_check(launch_axis(lp.logm, j, lp.blue, cols, variant(lp.logm, lp.blue, cols) != 0, stream),
               "axis pass launch");
 }
 
 void Context::enqueue_transform(const LinePlan& row, const LinePlan& col, int w, int h, bool inverse,
-                                const float2* src, float2* dst, hipStream_t stream) const
+                                const float2* src, float2* dst, float2* scratch, hipStream_t stream) const
 {
     const float2* s = src;
     if (w > 1) {
-        enqueue_axis(row, s, dst, 1, w, h, inverse, false, stream);
+        enqueue_axis(row, s, dst, 1, w, h, inverse, false, scratch, stream);
         s = dst;
     }
     if (h > 1) {
-        enqueue_axis(col, s, dst, w, 1, w, inverse, true, stream);
+        enqueue_axis(col, s, dst, w, 1, w, inverse, true, scratch, stream);
         s = dst;
     }
     if (s != dst)

@@ -42,7 +42,15 @@ struct LinePlan {
     bool blue = false;  // Bluestein (n not a power of two)
     float2* chirp = nullptr;  // n entries, Bluestein only
     float2* dtab = nullptr;   // m entries, Bluestein only
+    // lines longer than one tile: M = 2^logm1 * 2^logm2 (four-step), with a
+    // two-level table of exp(-2*pi*i*j/M)
+    int logm1 = 0, logm2 = 0;
+    float2* tw_lo = nullptr;
+    float2* tw_hi = nullptr;
     uint64_t bytes = 0;
+    bool large() const { return logm > MAX_LOGM; }
+    // scratch (complex<float> elements) one axis pass over `count` lines needs
+    uint64_t scratch_elems(int count) const { return large() ? (uint64_t)count * m * (blue ? 2 : 1) : 0; }
     ~LinePlan();
 };
 
@@ -60,12 +68,21 @@ public:
     std::shared_ptr<const LinePlan> line_plan(int n);
     uint64_t cached_bytes();
 
-    // Enqueue one axis pass of one image.
+    // Enqueue one axis pass of one image (scratch: lines longer than one tile).
     void enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
-                      bool inverse, bool cols, hipStream_t stream) const;
-    // Enqueue a whole w x h transform (rows then columns).
+                      bool inverse, bool cols, float2* scratch, hipStream_t stream) const;
+    // Enqueue a whole w x h transform (rows then columns); scratch must hold
+    // max(row.scratch_elems(h), col.scratch_elems(w)) elements.
     void enqueue_transform(const LinePlan& row, const LinePlan& col, int w, int h, bool inverse, const float2* src,
-                           float2* dst, hipStream_t stream) const;
+                           float2* dst, float2* scratch, hipStream_t stream) const;
+    // Four-step FFT of `count` lines of a long length (forward FFT of src into
+    // dst, both natural order), the building block of the long-line path.
+    void enqueue_large_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
+                            int64_t count, bool inverse, float2* scratch, hipStream_t stream) const;
+    // Forward power-of-two four-step FFT of length lp.m, times scale
+    // (scratch: count * m elements).
+    void enqueue_large_fft(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
+                           int64_t count, float scale, float2* scratch, hipStream_t stream) const;
 
     AxisJob make_job(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
                      bool inverse) const;

@@ -251,6 +251,46 @@ struct AxisJob {
 };
 
 // ---------------------------------------------------------------------------
+// image input and output of the view path: the luminance of an RGBA pixel
+// becomes the complex input (src/gpu/dft/code/dft_copy_input.comp:37-54) and
+// |X| * scale the R32F output (dft_copy_output.comp:34-48)
+// ---------------------------------------------------------------------------
+enum : int { IO_LUM = 1, IO_MAG = 2 };
+enum : int { PIXEL_RGBA32F = 0, PIXEL_RGBA8 = 1 };
+struct ImageIO {
+    const void* img;  // RGBA pixel at the rectangle's origin
+    int64_t iel;      // pixel stride along a line
+    int64_t ils;      // pixel stride between lines
+    int fmt;          // PIXEL_*
+    float mscale;     // 1/(w*h)
+    float* mag;       // R32F output at the rectangle's origin
+    int64_t mel;      // float stride along a line
+    int64_t mls;      // float stride between lines
+};
+
+// 0.2126 r + 0.7152 g + 0.0722 b (dft_copy_input.comp:37-40), fixed rounding order
+__host__ __device__ __forceinline__ float luminance_rgb(float r, float g, float b)
+{
+    return __builtin_fmaf(0.0722f, b, __builtin_fmaf(0.7152f, g, 0.2126f * r));
+}
+
+__device__ __forceinline__ float image_luminance(const ImageIO& io, int64_t idx)
+{
+    if (io.fmt == PIXEL_RGBA8) {
+        const uchar4 c = static_cast<const uchar4*>(io.img)[idx];
+        // UNORM: c / 255
+        return luminance_rgb((float)c.x / 255.0f, (float)c.y / 255.0f, (float)c.z / 255.0f);
+    }
+    const float4 c = static_cast<const float4*>(io.img)[idx];
+    return luminance_rgb(c.x, c.y, c.z);
+}
+
+__device__ __forceinline__ float cabs_scaled(float2 y, float s)
+{
+    return __builtin_sqrtf(__builtin_fmaf(y.x, y.x, y.y * y.y)) * s;
+}
+
+// ---------------------------------------------------------------------------
 // FFT of the tile held in registers
 // ---------------------------------------------------------------------------
 template <int LOGM, int THREADS, bool COLS, int P>
@@ -337,8 +377,11 @@ struct FftTile {
 // ---------------------------------------------------------------------------
 // one tile of one axis pass: load (+chirp), FFT, [x D, FFT], store (+chirp)
 // ---------------------------------------------------------------------------
-template <int LOGM, int THREADS, bool BLUE, bool COLS>
-__device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* lds, int tid)
+// IO (image kernels only): IO_LUM loads luminance from io.img instead of
+// complex values from job.src; IO_MAG stores |X| * io.mscale to io.mag
+// instead of complex values to job.dst.
+template <int LOGM, int THREADS, bool BLUE, bool COLS, int IO = 0>
+__device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* lds, int tid, ImageIO io = ImageIO{})
 {
     using C = LineCfg<LOGM, THREADS>;
     const int ll = COLS ? (tid % C::LPW) : (tid / C::TPL);
@@ -364,7 +407,10 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
             const uint32_t pos = uint32_t(t + i * C::TPL);
             float2 x = make_float2(0.0f, 0.0f);
             if (valid && in_line((int)pos)) {
-                x = s[i * step];
+                if constexpr ((IO & IO_LUM) != 0)
+                    x = make_float2(image_luminance(io, int64_t(line) * io.ils + int64_t(pos) * io.iel), 0.0f);
+                else
+                    x = s[i * step];
                 // forward x*c, inverse x*conj(c) (c = chirp); power of two: conj for the inverse
                 x = BLUE ? cmul(x, cxor(job.chirp[pos], sgn)) : cxor(x, sgn);
             }
@@ -399,7 +445,10 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
                 } else {
                     y = cscale(cxor(v[i], sgn), sc);
                 }
-                o[i * step] = y;
+                if constexpr ((IO & IO_MAG) != 0)
+                    io.mag[int64_t(line) * io.mls + int64_t(pos) * io.mel] = cabs_scaled(y, io.mscale);
+                else
+                    o[i * step] = y;
             }
         }
     }

@@ -1,0 +1,154 @@
+// Lines longer than one tile (padded length M > 16384): four-step FFT through
+// a scratch buffer, M = M1 * M2 with M1, M2 <= 16384 (the reference runs a
+// bit reversal, an LDS tile and log2(M/tile) global radix-2 stages here,
+// src/gpu/dft/fft.cpp:242-262, dft_fft_global.comp:44-78).
+//
+// With n = n1 + M1*n2 and k = k2 + M2*k1:
+//   step A: for every (line, n1): length-M2 FFT over n2, times w_M^(n1*k2),
+//           stored contiguously per (line, n1) in scratch;
+//   step C: for every (line, k2): length-M1 FFT over n1, stored at k2 + M2*k1
+//           (natural order).
+// A power-of-two line is A then C.  A Bluestein line is A (chirp and zero pad
+// on the load), C (times D, conjugated, into a padded buffer), A, C (conjugate,
+// chirp, crop, 1/n on the store).  Every step is a strided tile of whole
+// sub-lines run with the column (lines-fastest) mapping.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_device.hpp"
+#include "launch.hpp"
+
+namespace mdft {
+
+constexpr int LARGE_T = 1024;
+
+// line L of a step maps to (L / sub) * ls + (L % sub) * ls2; element p to p * es
+__device__ __forceinline__ int64_t line_base(int64_t L, int sub, int64_t ls, int64_t ls2)
+{
+    return (L / sub) * ls + (L % sub) * ls2;
+}
+
+template <int LOGM>
+__global__ __launch_bounds__(LARGE_T) void k_large_step(LargeJob job)
+{
+    using C = LineCfg<LOGM, LARGE_T>;
+    constexpr int LF = C::template lds_floats<true>();
+    __shared__ float lds[LF > 0 ? LF : 1];
+    const int tid = threadIdx.x;
+    const int ll = tid % C::LPW;
+    const int t = tid / C::LPW;
+    const int64_t line = int64_t(xcd_remap(blockIdx.x, gridDim.x)) * C::LPW + ll;
+    const bool valid = line < job.count;
+    const uint32_t sgn = job.inverse ? 0x80000000u : 0u;
+    // sub-line coordinate inside the long line: n1 (step A) or k2 (step C)
+    const int64_t sub_idx = line % job.sub_in;
+
+    float2 v[C::EPT];
+    {
+        const float2* s = job.src + (valid ? line_base(line, job.sub_in, job.ls_in, job.ls2_in) : 0);
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int64_t pos = t + i * C::TPL;
+            float2 x = make_float2(0.0f, 0.0f);
+            if (valid) {
+                if (job.load_mode == LARGE_LOAD_CHIRP) {
+                    // element n1 + M1*n2 of the Bluestein input: chirped, zero past n
+                    const int64_t j = sub_idx + job.m1 * pos;
+                    if (j < job.n) x = cmul(s[pos * job.es_in], cxor(job.chirp[j], sgn));
+                } else if (job.load_mode == LARGE_LOAD_CONJ) {
+                    x = cxor(s[pos * job.es_in], sgn);
+                } else {
+                    x = s[pos * job.es_in];
+                }
+            }
+            v[i] = x;
+        }
+    }
+
+    FftTile<LOGM, LARGE_T, true>::run(v, t, ll, lds, job.tw);
+
+    if (!valid) return;
+    float2* o = job.dst + line_base(line, job.sub_out, job.ls_out, job.ls2_out);
+    const int64_t q_line = line % job.sub_out;
+#pragma unroll
+    for (int i = 0; i < C::EPT; ++i) {
+        const int64_t pos = t + i * C::TPL;
+        float2 y = v[i];
+        if (job.store_mode == LARGE_STORE_TWIDDLE) {
+            // times w_M^(n1*k2), n1*k2 < M1*M2 = M: two-level table
+            const int64_t q = q_line * pos;
+            y = cmul(y, cmul(job.tw_lo[q & (LARGE_LO - 1)], job.tw_hi[q >> LARGE_LO_BITS]));
+        } else if (job.store_mode == LARGE_STORE_DIAG) {
+            // natural index k = k2 + M2*k1; forward conj(v*D), inverse conj(v*conj(D))
+            const int64_t k = q_line + job.m2 * pos;
+            y = cmul_conj(y, cxor(job.dtab[k], sgn));
+        } else if (job.store_mode == LARGE_STORE_CHIRP) {
+            const int64_t k = q_line + job.m2 * pos;
+            if (k >= job.n) continue;
+            y = cscale(cmul_conj(y, cxor(job.chirp[k], sgn ^ 0x80000000u)), job.scale);
+        } else if (job.store_mode == LARGE_STORE_CONJ) {
+            y = cscale(cxor(y, sgn), job.scale);
+        }
+        o[pos * job.es_out] = y;
+    }
+}
+
+template <int LOGM>
+static hipError_t launch_large_t(const LargeJob& job, hipStream_t stream)
+{
+    constexpr int LPW = LineCfg<LOGM, LARGE_T>::LPW;
+    const int64_t grid = (job.count + LPW - 1) / LPW;
+    if (grid <= 0) return hipSuccess;
+    if (grid > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_large_step<LOGM>, dim3((unsigned)grid), dim3(LARGE_T), 0, stream, job);
+    return hipGetLastError();
+}
+
+hipError_t launch_large_step(int logm, const LargeJob& job, hipStream_t stream)
+{
+    switch (logm) {
+    case 1: return launch_large_t<1>(job, stream);
+    case 2: return launch_large_t<2>(job, stream);
+    case 3: return launch_large_t<3>(job, stream);
+    case 4: return launch_large_t<4>(job, stream);
+    case 5: return launch_large_t<5>(job, stream);
+    case 6: return launch_large_t<6>(job, stream);
+    case 7: return launch_large_t<7>(job, stream);
+    case 8: return launch_large_t<8>(job, stream);
+    case 9: return launch_large_t<9>(job, stream);
+    case 10: return launch_large_t<10>(job, stream);
+    case 11: return launch_large_t<11>(job, stream);
+    case 12: return launch_large_t<12>(job, stream);
+    case 13: return launch_large_t<13>(job, stream);
+    case 14: return launch_large_t<14>(job, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// exp(-2*pi*i*j/M) for j < 2^LARGE_LO_BITS, and exp(-2*pi*i*j*2^LARGE_LO_BITS/M)
+// for j < M / 2^LARGE_LO_BITS, in FP64 then rounded
+__global__ void k_gen_large_twiddles(float2* lo, float2* hi, int64_t m)
+{
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t nhi = m >> LARGE_LO_BITS;
+    if (j < LARGE_LO) {
+        double s, c;
+        sincospi(2.0 * (double)j / (double)m, &s, &c);
+        lo[j] = make_float2((float)c, (float)-s);
+    }
+    if (j < nhi) {
+        double s, c;
+        sincospi(2.0 * (double)(j << LARGE_LO_BITS) / (double)m, &s, &c);
+        hi[j] = make_float2((float)c, (float)-s);
+    }
+}
+
+hipError_t gen_large_twiddles(float2* lo, float2* hi, int64_t m, hipStream_t stream)
+{
+    const int64_t nhi = m >> LARGE_LO_BITS;
+    const int64_t n = nhi > LARGE_LO ? nhi : LARGE_LO;
+    hipLaunchKernelGGL(k_gen_large_twiddles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, lo, hi, m);
+    return hipGetLastError();
+}
+
+}  // namespace mdft

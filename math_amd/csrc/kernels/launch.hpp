@@ -26,11 +26,49 @@ constexpr uint32_t BATCH_CTL_HEADER = 4;
 // item.y flag: a whole small image, item.x = row job, item.y & ~flag = column job
 constexpr uint32_t BATCH_WHOLE = 0x80000000u;
 
+// ---- lines longer than one tile (four-step through a scratch buffer) ----
+constexpr int LARGE_LO_BITS = 14;
+constexpr int64_t LARGE_LO = int64_t(1) << LARGE_LO_BITS;
+constexpr int LARGE_MAX_LOGM = 2 * MAX_LOGM;  // M1, M2 <= 2^14
+enum LargeLoad { LARGE_LOAD_PLAIN = 0, LARGE_LOAD_CONJ = 1, LARGE_LOAD_CHIRP = 2 };
+enum LargeStore { LARGE_STORE_PLAIN = 0, LARGE_STORE_TWIDDLE = 1, LARGE_STORE_DIAG = 2, LARGE_STORE_CHIRP = 3,
+                  LARGE_STORE_CONJ = 4 };
+
+// One step of a long-line FFT: count sub-lines; sub-line L reads from
+// src + (L / sub_in) * ls_in + (L % sub_in) * ls2_in with element stride es_in
+// and writes likewise with the *_out fields.
+struct LargeJob {
+    const float2* src;
+    float2* dst;
+    int64_t ls_in, ls2_in, es_in, sub_in;
+    int64_t ls_out, ls2_out, es_out, sub_out;
+    int64_t count;
+    int load_mode, store_mode, inverse, pad;
+    int64_t n, m1, m2;
+    float scale;
+    const float2* chirp;
+    const float2* dtab;
+    const float2* tw_lo;
+    const float2* tw_hi;
+    const float2* tw;  // Stockham rows of the sub-FFT length
+};
+
+hipError_t launch_large_step(int logm, const LargeJob& job, hipStream_t stream);
+hipError_t gen_large_twiddles(float2* lo, float2* hi, int64_t m, hipStream_t stream);
+hipError_t gen_h2_f32(float2* h2, const double2* chirp64, int64_t n, int64_t m, hipStream_t stream);
+
 hipError_t gen_twiddles(float2* table, hipStream_t stream);
 hipError_t gen_chirp(double2* chirp64, float2* chirp32, int n, hipStream_t stream);
 hipError_t gen_dtab(float2* dtab, const double2* chirp64, double* cos_scratch, int n, int m, hipStream_t stream);
 
 hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, bool wide, hipStream_t stream);
+
+// image path: an axis pass with fused luminance input and/or magnitude output
+// (mode = IO_LUM | IO_MAG bits), and the unfused conversions
+hipError_t launch_image_axis(int logm, const AxisJob& job, const ImageIO& io, bool blue, bool cols, int mode,
+                             bool wide, hipStream_t stream);
+hipError_t launch_image_luminance(const ImageIO& io, float2* dst, int w, int h, hipStream_t stream);
+hipError_t launch_image_magnitude(const ImageIO& io, const float2* src, int w, int h, hipStream_t stream);
 
 // batch kernels: threads is 256 (padded lengths up to 4096) or 1024 (up to 8192)
 hipError_t launch_batch_grid(int threads, const BatchJob* jobs, const uint2* items, uint32_t nitems,

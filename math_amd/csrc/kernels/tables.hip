@@ -86,7 +86,27 @@ __global__ void __launch_bounds__(256) k_gen_dtab(float2* dtab, const double2* c
     }
 }
 
+// Circulant embedding of conj(chirp) rounded to FP32, for lengths whose
+// diagonal is computed by an FFT instead of the direct sum.
+__global__ void k_gen_h2_f32(float2* h2, const double2* chirp64, int64_t n, int64_t m)
+{
+    const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (l >= m) return;
+    double2 c = make_double2(0.0, 0.0);
+    if (l < n)
+        c = chirp64[l];
+    else if (l > m - n)
+        c = chirp64[m - l];
+    h2[l] = make_float2((float)c.x, (float)-c.y);
+}
+
 // --- host launchers --------------------------------------------------------
+
+hipError_t gen_h2_f32(float2* h2, const double2* chirp64, int64_t n, int64_t m, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_gen_h2_f32, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, h2, chirp64, n, m);
+    return hipGetLastError();
+}
 
 hipError_t gen_twiddles(float2* table, hipStream_t stream)
 {
@@ -105,7 +125,8 @@ hipError_t gen_twiddles(float2* table, hipStream_t stream)
 
 hipError_t gen_chirp(double2* chirp64, float2* chirp32, int n, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_gen_chirp, dim3((n + 255) / 256), dim3(256), 0, stream, chirp64, chirp32, n);
+    hipLaunchKernelGGL(k_gen_chirp, dim3((unsigned)((int64_t(n) + 255) / 256)), dim3(256), 0, stream, chirp64, chirp32,
+                       n);
     return hipGetLastError();
 }
 

@@ -104,6 +104,15 @@ def lib():
             "mdft_exec_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
             "mdft_tune_get": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
             "mdft_tune_set": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+            "mdft_fft_create": ([ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int),
+            "mdft_fft_exec": ([_vp, ctypes.c_int, _vp, ctypes.c_size_t], ctypes.c_int),
+            "mdft_fft_exec_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+            "mdft_fft_destroy": ([_vp], ctypes.c_int),
+            "mdft_image_create": ([ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32],
+                                  ctypes.c_int),
+            "mdft_image_exec_device": ([_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                        ctypes.c_uint32, _vp, ctypes.c_uint64, _vp], ctypes.c_int),
+            "mdft_image_destroy": ([_vp], ctypes.c_int),
             "mdft_batch_create": ([ctypes.POINTER(_vp), ctypes.POINTER(mdft_desc), ctypes.c_uint32, ctypes.c_uint32],
                                   ctypes.c_int),
             "mdft_batch_exec_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
@@ -230,6 +239,71 @@ class DFT:
 def create_hip(x, y):
     """The HIP counterpart of create_fftw(x, y) / create_cufft(x, y) (src/dft/fftw.h:28)."""
     return DFT(x, y)
+
+
+class Fft:
+    """count power-of-two FFTs of length n back to back (mdft_fft_create); the
+    shape of ns::gpu::dft::Fft (src/gpu/dft/fft.h:32-53)."""
+
+    def __init__(self, count, n):
+        self._h = _vp()
+        _check(lib().mdft_fft_create(ctypes.byref(self._h), int(count), int(n)))
+        self.count, self.n = int(count), int(n)
+
+    def run_for_data(self, inverse, data):
+        """In place on a C-contiguous complex64 numpy array of count*n values."""
+        if not isinstance(data, np.ndarray) or data.dtype != np.complex64 or not data.flags.c_contiguous:
+            raise ErrorException(MDFT_ERR_DATA_SIZE, "FFT buffer must be a contiguous complex64 array")
+        _check(lib().mdft_fft_exec(self._h, int(bool(inverse)), data.ctypes.data, data.size))
+
+    def exec_device(self, inverse, src_ptr, dst_ptr, stream=0):
+        _check(lib().mdft_fft_exec_device(self._h, int(bool(inverse)), src_ptr, dst_ptr, stream))
+
+    def close(self):
+        if self._h:
+            lib().mdft_fft_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def create_fft(count, n):
+    """create_fft(device, family_indices, count, n) (src/gpu/dft/fft.h:49-53)."""
+    return Fft(count, n)
+
+
+PIXEL_RGBA32F = 0
+PIXEL_RGBA8 = 1
+
+
+class Image:
+    """The view's image path (mdft_image_*): luminance of an RGBA rectangle,
+    forward 2-D DFT, |X|/(w*h) into an R32F image; the shape of
+    ns::gpu::dft::ComputeImage (src/gpu/dft/compute.h:35-50)."""
+
+    def __init__(self, width, height, pixel_format=PIXEL_RGBA32F):
+        self._h = _vp()
+        _check(lib().mdft_image_create(ctypes.byref(self._h), int(width), int(height), int(pixel_format)))
+        self.width, self.height = int(width), int(height)
+
+    def exec_device(self, src_ptr, src_width, src_height, src_pitch, x, y, dst_ptr, dst_pitch, stream=0):
+        _check(lib().mdft_image_exec_device(self._h, src_ptr, int(src_width), int(src_height), int(src_pitch), int(x),
+                                            int(y), dst_ptr, int(dst_pitch), stream))
+
+    def close(self):
+        if self._h:
+            lib().mdft_image_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

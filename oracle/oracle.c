@@ -660,3 +660,30 @@ void orc_fill(uint64_t seed, int dist, size_t count, float* out)
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Image path: luminance of the source rectangle                             */
+/* (src/gpu/dft/code/dft_copy_input.comp:37-54)                              */
+/* ------------------------------------------------------------------------ */
+
+void orc_image_luminance(int fmt, const void* src, int64_t pitch, int x, int y, int w, int h, float* lum)
+{
+    for (int j = 0; j < h; ++j) {
+        for (int i = 0; i < w; ++i) {
+            const int64_t idx = (int64_t)(y + j) * pitch + (x + i);
+            float r, g, b;
+            if (fmt == 1) {
+                const unsigned char* p = (const unsigned char*)src + 4 * idx;
+                r = (float)p[0] / 255.0f;
+                g = (float)p[1] / 255.0f;
+                b = (float)p[2] / 255.0f;
+            } else {
+                const float* p = (const float*)src + 4 * idx;
+                r = p[0];
+                g = p[1];
+                b = p[2];
+            }
+            lum[(int64_t)j * w + i] = fmaf(0.0722f, b, fmaf(0.7152f, g, 0.2126f * r));
+        }
+    }
+}

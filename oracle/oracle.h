@@ -86,6 +86,14 @@ void orc_error_metrics(size_t count, const float* got, const double* ref, double
  * Deterministic for (seed, index); 2*count floats. */
 void orc_fill(uint64_t seed, int dist, size_t count, float* out);
 
+/* Image path of the view (src/gpu/dft/compute.cpp:91-200): the luminance
+ * 0.2126 r + 0.7152 g + 0.0722 b of each pixel of the w x h rectangle at
+ * (x, y) of an RGBA image with a row pitch of `pitch` pixels
+ * (dft_copy_input.comp:37-54; fmt 0 = 4 floats, fmt 1 = 4 x uint8 UNORM),
+ * rounded in the order fma(0.0722, b, fma(0.7152, g, 0.2126 * r)), into
+ * lum (w*h floats). */
+void orc_image_luminance(int fmt, const void* src, int64_t pitch, int x, int y, int w, int h, float* lum);
+
 #ifdef __cplusplus
 }
 #endif

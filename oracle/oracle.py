@@ -43,6 +43,8 @@ def lib():
         L.orc_dft2d_64_from32.restype = ctypes.c_int
         L.orc_error_metrics.argtypes = [ctypes.c_size_t, _fp, _dp, _dp]
         L.orc_fill.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t, _fp]
+        L.orc_image_luminance.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, _fp]
         _LIB = L
     return _LIB
 
@@ -162,6 +164,25 @@ def fill(seed, dist, shape):
     out = np.empty(shape, np.complex64)
     lib().orc_fill(seed, 0 if dist == "uniform" else 1, out.size, out.ctypes.data_as(_fp))
     return out
+
+
+def image_luminance(pixels, x, y, w, h):
+    """Luminance of the w x h rectangle at (x, y) of an RGBA image given as a
+    (rows, cols, 4) float32 or uint8 array (orc_image_luminance)."""
+    pixels = np.ascontiguousarray(pixels)
+    fmt = 1 if pixels.dtype == np.uint8 else 0
+    if fmt == 0 and pixels.dtype != np.float32:
+        raise ValueError("pixels must be float32 or uint8")
+    out = np.empty((h, w), np.float32)
+    lib().orc_image_luminance(fmt, pixels.ctypes.data, pixels.shape[1], x, y, w, h, out.ctypes.data_as(_fp))
+    return out
+
+
+def image_dft_64(pixels, x, y, w, h, nthreads=1):
+    """The view's image path in FP64 after the FP32 luminance: |DFT(lum)| / (w*h)
+    (src/gpu/dft/compute.cpp:146-176, dft_copy_output.comp:34-48)."""
+    lum = image_luminance(pixels, x, y, w, h).astype(np.complex128)
+    return np.abs(dft2d_64(lum, False, nthreads=nthreads)) / (float(w) * float(h))
 
 
 def tolerance(w, h):

@@ -70,3 +70,29 @@ def test_null_out_pointers():
 def test_create_hip_rejects_non_positive():
     with pytest.raises(M.ErrorException):
         M.create_hip(0, 3)
+
+
+def test_fft_size_errors_use_reference_text():
+    # src/gpu/dft/fft.cpp:298-306, checked before any device work
+    h = ctypes.c_void_p()
+    assert M.lib().mdft_fft_create(ctypes.byref(h), 4, 0) == M.MDFT_ERR_SIZE
+    assert M.lib().mdft_last_error() == b"FFT size 0 is not positive"
+    assert M.lib().mdft_fft_create(ctypes.byref(h), 4, 12) == M.MDFT_ERR_SIZE
+    assert M.lib().mdft_last_error() == b"FFT size 12 is not an integral power of 2"
+    assert M.lib().mdft_fft_create(ctypes.byref(h), 4, 1 << 29) == M.MDFT_ERR_SIZE
+    assert b"longest supported line" in M.lib().mdft_last_error()
+    assert M.lib().mdft_fft_create(None, 4, 8) == M.MDFT_ERR_INVALID
+    assert M.lib().mdft_fft_destroy(None) == M.MDFT_OK
+    d = np.zeros(8, np.complex64)
+    assert M.lib().mdft_fft_exec(None, 0, d.ctypes.data, 8) == M.MDFT_ERR_NO_BUFFERS
+    with pytest.raises(M.ErrorException, match="not an integral power of 2"):
+        M.create_fft(3, 6)
+
+
+def test_image_create_rejects_bad_format_and_size():
+    h = ctypes.c_void_p()
+    assert M.lib().mdft_image_create(ctypes.byref(h), 8, 8, 7) == M.MDFT_ERR_INVALID
+    assert b"pixel format" in M.lib().mdft_last_error()
+    assert M.lib().mdft_image_create(ctypes.byref(h), 0, 8, 0) == M.MDFT_ERR_SIZE
+    assert M.lib().mdft_last_error() == b"Error size 0 for compute m"
+    assert M.lib().mdft_image_destroy(None) == M.MDFT_OK

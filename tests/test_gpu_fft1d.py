@@ -1,0 +1,108 @@
+"""GPU parity of the batched power-of-two 1-D FFT (mdft_fft_*), the
+reference's Fft component (src/gpu/dft/fft.h:32-53, fft.cpp:207-330), against
+the FP64 oracle FFT: count lines of length n back to back, forward
+unnormalised, inverse scaled by 1/n.  Bound: normalised L2 <= 2e-6*log2(n)
+per call, as for the 2-D path."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import rand_c64
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mdft():
+    from math_amd import mdft as M
+
+    M.init(0)
+    return M
+
+
+def tol(n):
+    return 2e-6 * max(1.0, math.log2(n))
+
+
+def ref_lines(x, inverse):
+    count, n = x.shape
+    out = np.empty(x.shape, np.complex128)
+    for j in range(count):
+        out[j] = O.dft1d_64(x[j], inverse)
+        if inverse:
+            out[j] /= n
+    return out
+
+
+@pytest.mark.parametrize("count,n", [(1, 2), (5, 4), (7, 16), (3, 64), (100, 256), (9, 1024), (2, 4096), (3, 8192),
+                                     (2, 16384), (2, 32768), (1, 1 << 18), (1000, 8), (333, 32)])
+def test_lines_match_fp64(mdft, count, n):
+    rng = np.random.default_rng(count * 7919 + n)
+    x = rand_c64(rng, (count, n))
+    f = mdft.create_fft(count, n)
+    for inv in (False, True):
+        d = x.copy().ravel()
+        f.run_for_data(inv, d)
+        m = O.error_metrics(d, ref_lines(x, inv).ravel())
+        assert m["nl2"] <= tol(n), (count, n, inv, m)
+    f.close()
+
+
+def test_round_trip_and_impulse(mdft):
+    n, count = 2048, 6
+    rng = np.random.default_rng(3)
+    x = rand_c64(rng, (count, n))
+    f = mdft.Fft(count, n)
+    d = x.copy().ravel()
+    f.run_for_data(False, d)
+    f.run_for_data(True, d)
+    m = O.error_metrics(d, x.astype(np.complex128).ravel())
+    assert m["nl2"] <= tol(n), m
+    # a unit impulse at position 1 transforms to exp(-2*pi*i*k/n)
+    e = np.zeros((count, n), np.complex64)
+    e[:, 1] = 1
+    d = e.ravel().copy()
+    f.run_for_data(False, d)
+    k = np.arange(n)
+    expect = np.tile(np.exp(-2j * np.pi * k / n), count)
+    assert np.abs(d - expect).max() < 1e-6
+
+
+def test_size_one_is_a_no_op(mdft):
+    f = mdft.Fft(5, 1)
+    d = np.arange(5, dtype=np.complex64) + 1j
+    before = d.copy()
+    f.run_for_data(True, d)
+    assert np.array_equal(d, before)
+    # the reference returns before its size check for n == 1 (fft.cpp:271-274)
+    f.run_for_data(False, np.zeros(3, np.complex64))
+
+
+def test_wrong_buffer_size_message(mdft):
+    f = mdft.Fft(4, 8)
+    with pytest.raises(mdft.ErrorException, match="FFT buffer size 31 must be equal to data size 32"):
+        f.run_for_data(False, np.zeros(31, np.complex64))
+
+
+def test_device_out_of_place_and_bitwise_equal_to_rows(mdft):
+    import torch
+
+    count, n = 37, 512
+    rng = np.random.default_rng(17)
+    x = rand_c64(rng, (count, n))
+    src = torch.from_numpy(x.copy().ravel()).cuda()
+    dst = torch.empty_like(src)
+    f = mdft.Fft(count, n)
+    s = torch.cuda.current_stream().cuda_stream
+    f.exec_device(False, src.data_ptr(), dst.data_ptr(), s)
+    torch.cuda.synchronize()
+    assert np.array_equal(src.cpu().numpy(), x.ravel())
+    got = dst.cpu().numpy()
+    # a width-n, height-1 2-D plan runs the same row kernel on one line
+    p = mdft.Plan(n, 1)
+    for j in range(3):
+        line = x[j].copy()
+        p.exec(False, line)
+        assert np.array_equal(line, got[j * n:(j + 1) * n])

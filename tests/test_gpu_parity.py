@@ -229,3 +229,19 @@ def test_host_and_device_paths_bitwise_equal(mdft):
     p.exec_device(False, t.data_ptr(), t.data_ptr(), torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     assert np.array_equal(a, t.cpu().numpy())
+
+
+@pytest.mark.parametrize("w,h", [(16385, 2), (2, 16385), (32768, 3), (3, 32768), (65536, 1), (1, 20000),
+                                 (40000, 1), (9000, 5)])
+def test_lines_longer_than_one_tile(mdft, w, h):
+    # padded length above 16384: the four-step path through plan-owned scratch
+    rng = np.random.default_rng(w + 7 * h)
+    x = rand_c64(rng, (h, w))
+    log(f"long line {w}x{h}")
+    for inv in (False, True):
+        check_against_fp64(run_exec(mdft, x, inv), x, inv, "four-step")
+
+
+def test_longest_line_limit_message(mdft):
+    with pytest.raises(mdft.ErrorException, match="exceeds the longest supported line"):
+        mdft.Plan((1 << 27) + 3, 1)

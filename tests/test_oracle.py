@@ -234,3 +234,47 @@ def test_fill_deterministic_and_ranges():
     assert a.real.min() >= -1 and a.real.max() < 1
     n = O.fill(2, "normal", (300, 300))
     assert abs(n.real.mean()) < 0.02 and abs(n.real.std() - 1) < 0.02
+
+
+# ---- image path (src/gpu/dft/compute.cpp:91-200) ----------------------------
+
+def test_image_luminance_weights():
+    # dft_copy_input.comp:37-40: pure primaries give the weights, white gives their sum
+    px = np.zeros((1, 4, 4), np.float32)
+    px[0, 0, :3] = (1, 0, 0)
+    px[0, 1, :3] = (0, 1, 0)
+    px[0, 2, :3] = (0, 0, 1)
+    px[0, 3, :3] = (1, 1, 1)
+    lum = O.image_luminance(px, 0, 0, 4, 1)[0]
+    assert np.allclose(lum, [0.2126, 0.7152, 0.0722, 1.0], rtol=0, atol=1e-7)
+    # UNORM bytes: 255 is 1.0, alpha is ignored
+    p8 = np.zeros((1, 2, 4), np.uint8)
+    p8[0, 0] = (255, 255, 255, 0)
+    p8[0, 1] = (51, 0, 0, 255)
+    lum8 = O.image_luminance(p8, 0, 0, 2, 1)[0]
+    assert abs(lum8[0] - 1.0) < 1e-7 and abs(lum8[1] - 0.2126 * 0.2) < 1e-7
+
+
+def test_image_luminance_rectangle_offset():
+    rng = np.random.default_rng(4)
+    px = rng.uniform(0, 1, (9, 11, 4)).astype(np.float32)
+    lum = O.image_luminance(px, 3, 2, 5, 4)
+    sub = px[2:6, 3:8].astype(np.float64)
+    expect = 0.2126 * sub[..., 0] + 0.7152 * sub[..., 1] + 0.0722 * sub[..., 2]
+    assert np.abs(lum - expect).max() < 1e-6
+
+
+def test_image_dft_closed_forms():
+    w, h = 6, 5
+    # constant luminance l: only the DC term, |X(0,0)|/(w*h) = l
+    px = np.zeros((h, w, 4), np.float32)
+    px[..., :3] = 0.5
+    out = O.image_dft_64(px, 0, 0, w, h)
+    assert abs(out[0, 0] - 0.5) < 1e-7
+    out[0, 0] = 0
+    assert np.abs(out).max() < 1e-12
+    # an impulse of luminance l has a flat spectrum l/(w*h)
+    px = np.zeros((h, w, 4), np.float32)
+    px[0, 0, :3] = 1.0
+    out = O.image_dft_64(px, 0, 0, w, h)
+    assert np.abs(out - 1.0 / (w * h)).max() < 1e-7

@@ -94,7 +94,31 @@ __host__ __device__ constexpr int64_t tw_off(int logm)
 }
 __host__ __device__ constexpr int64_t tw_total() { return tw_off(MAX_LOGM + 1); }
 
-__host__ __device__ constexpr int pad_pos(int p) { return p + (p >> 4); }
+#ifndef MDFT_LDS_LAYOUT
+#define MDFT_LDS_LAYOUT 1
+#endif
+// LDS word of line position p in the exchange written by a pass of stride NS.
+// A pass writes its 16 outputs NS positions apart and the next pass reads 16
+// positions TPL apart.  When a 32-lane half-wave covers 32 consecutive
+// positions of one line (ONE_LINE), the reads are conflict-free unpadded and
+// the writes need: NS = 1 (lanes 16 apart) one pad word per 32, NS = 16
+// (lanes in two runs of 16, 256 apart) 16 pad words per 256, NS >= 256
+// (consecutive lanes) none.  Otherwise lanes span several lines and one pad
+// word per 16 with an odd line stride keeps them apart.  Every map satisfies
+// f(t + i*TPL) = f(t) + f(i*TPL) for t < TPL, so LDS offsets are immediates.
+template <int NS, bool ONE_LINE>
+__host__ __device__ constexpr int xpad(int p)
+{
+    if constexpr (!ONE_LINE || !MDFT_LDS_LAYOUT) {
+        return p + (p >> 4);
+    } else if constexpr (NS == 1) {
+        return p + (p >> 5);
+    } else if constexpr (NS == 16) {
+        return p + 16 * (p >> 8);
+    } else {
+        return p;
+    }
+}
 
 template <int LOGM, int THREADS>
 struct LineCfg {
@@ -106,6 +130,12 @@ struct LineCfg {
     static_assert(TPL <= THREADS, "line does not fit the workgroup");
     static_assert(LPW * TPL == THREADS, "workgroup must hold whole lines");
     static constexpr int PADM = M + (M >> 4);
+    // a 32-lane half-wave covers 32 consecutive positions of one line
+    template <bool COLS>
+    static constexpr bool one_line()
+    {
+        return COLS ? LPW == 1 : TPL >= 32;
+    }
     // Line stride in LDS words.  Lanes of one wave cover q consecutive
     // positions of 32/q lines (rows: q = TPL; columns, lines fastest:
     // q = 32/LPW); a stride of q times an odd number puts them on distinct
@@ -293,26 +323,81 @@ __device__ __forceinline__ float cabs_scaled(float2 y, float s)
 // ---------------------------------------------------------------------------
 // FFT of the tile held in registers
 // ---------------------------------------------------------------------------
+#ifndef MDFT_WAVE_LOCAL_SYNC
+#define MDFT_WAVE_LOCAL_SYNC 1
+#endif
+// Barrier around an LDS exchange.  When every line of a tile lies inside one
+// wave (row mapping with TPL <= 64), an exchange never crosses waves, and the
+// LDS operations of one wave are performed in order, so a wave-level
+// code-motion barrier replaces the workgroup barrier: waves of the tile then
+// run their passes independently.
+template <bool WAVE>
+__device__ __forceinline__ void exchange_sync()
+{
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+#ifndef MDFT_TW_PREFETCH
+#define MDFT_TW_PREFETCH 1
+#endif
+// Twiddle rows of pass P for the thread's butterflies: NB rows of R/2 float4
+// (EPT/2 in all).  Loaded one pass ahead (pass 1's with the tile's input), so
+// the table latency overlaps the LDS exchange instead of following it.
+// Off for padded lengths above 4096 (four passes), where the extra registers
+// spill at 1024 threads.
+template <int LOGM>
+__host__ __device__ constexpr bool tw_prefetch()
+{
+    return MDFT_TW_PREFETCH && LOGM <= 12;
+}
+
+template <int LOGM, int THREADS, int P>
+struct TwLoad {
+    using C = LineCfg<LOGM, THREADS>;
+    static constexpr int R = 1 << pass_bits(LOGM, P);
+    static constexpr int NB = C::EPT / R;
+    static constexpr int NS = 1 << (4 * P);
+    __device__ __forceinline__ static void run(float4 (&w)[C::EPT / 2], int t, const float2* tw)
+    {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int k = (t + b * C::TPL) & (NS - 1);
+            const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * R);
+#pragma unroll
+            for (int r2 = 0; r2 < R / 2; ++r2) w[b * (R / 2) + r2] = row[r2];
+        }
+    }
+};
+
 template <int LOGM, int THREADS, bool COLS, int P>
 struct FftPass {
     using C = LineCfg<LOGM, THREADS>;
+    static constexpr bool WAVE = MDFT_WAVE_LOCAL_SYNC && !COLS && C::TPL <= 64;
     static constexpr int RB = pass_bits(LOGM, P);
     static constexpr int R = 1 << RB;
     static constexpr int NB = C::EPT / R;
     static constexpr int NS = 1 << (4 * P);
     static constexpr bool LAST = (P == C::NPASS - 1);
     static constexpr int LST = C::template lstride<COLS>();
+    static constexpr bool ONE = C::template one_line<COLS>();
 
-    __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, float* re, float* im, const float2* tw)
+    __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, float* re, float* im, const float2* tw,
+                                               float4 (&w)[C::EPT / 2])
     {
         if constexpr (P > 0) {
-            __syncthreads();
-            // when TPL is a multiple of 16, pad_pos(t + i*TPL) is pad_pos(t)
-            // plus a compile-time offset (an LDS immediate)
-            const int q0 = pad_pos(t);
+            exchange_sync<WAVE>();
+            // layout of the previous pass (stride NS/16); the offset of slot i
+            // is a compile-time constant (an LDS immediate)
+            const int q0 = xpad<NS / 16, ONE>(t);
 #pragma unroll
             for (int i = 0; i < C::EPT; ++i) {
-                const int q = (C::TPL % 16 == 0) ? q0 + i * (C::TPL + C::TPL / 16) : pad_pos(t + i * C::TPL);
+                const int q = q0 + xpad<NS / 16, ONE>(i * C::TPL);
                 v[i] = make_float2(re[q], im[q]);
             }
         }
@@ -328,7 +413,7 @@ struct FftPass {
                 const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * R);
 #pragma unroll
                 for (int r2 = 0; r2 < R / 2; ++r2) {
-                    const float4 q = row[r2];
+                    const float4 q = tw_prefetch<LOGM>() ? w[b * (R / 2) + r2] : row[r2];
                     if (r2 > 0) u[2 * r2] = cmul(u[2 * r2], make_float2(q.x, q.y));
                     u[2 * r2 + 1] = cmul(u[2 * r2 + 1], make_float2(q.z, q.w));
                 }
@@ -338,18 +423,20 @@ struct FftPass {
             for (int r = 0; r < R; ++r) v[b + NB * r] = u[r];
         }
         if constexpr (!LAST) {
-            __syncthreads();
+            if constexpr (tw_prefetch<LOGM>()) TwLoad<LOGM, THREADS, P + 1>::run(w, t, tw);
+            exchange_sync<WAVE>();
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int j = t + b * C::TPL;
                 const int k = j & (NS - 1);
                 const int base = (j - k) * R + k;
                 // NS == 1 (first pass, R == 16): base is a multiple of 16, so
-                // the 16 outputs are contiguous after padding; NS >= 16: NS is
-                // a multiple of 16, so each step adds NS + NS/16
+                // the 16 outputs are contiguous after padding; NS >= 16: the
+                // outputs lie in one padding block apart from whole blocks, so
+                // each step adds xpad(NS)
                 static_assert(NS == 1 ? R == 16 : NS % 16 == 0, "linear write offsets");
-                constexpr int step = (NS == 1) ? 1 : NS + NS / 16;
-                const int q0 = pad_pos(base);
+                constexpr int step = xpad<NS, ONE>(NS);
+                const int q0 = xpad<NS, ONE>(base);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int q = q0 + r * step;
@@ -357,7 +444,7 @@ struct FftPass {
                     im[q] = v[b + NB * r].y;
                 }
             }
-            FftPass<LOGM, THREADS, COLS, P + 1>::run(v, t, re, im, tw);
+            FftPass<LOGM, THREADS, COLS, P + 1>::run(v, t, re, im, tw, w);
         }
     }
 };
@@ -370,7 +457,9 @@ struct FftTile {
         constexpr int LST = C::template lstride<COLS>();
         float* re = lds + ll * LST;
         float* im = lds + C::LPW * LST + ll * LST;
-        FftPass<LOGM, THREADS, COLS, 0>::run(v, t, re, im, tw);
+        float4 w[C::EPT / 2];
+        if constexpr (tw_prefetch<LOGM>() && C::NPASS > 1) TwLoad<LOGM, THREADS, 1>::run(w, t, tw);
+        FftPass<LOGM, THREADS, COLS, 0>::run(v, t, re, im, tw, w);
     }
 };
 

@@ -351,16 +351,32 @@ __device__ __forceinline__ void exchange_sync()
 // the table latency overlaps the LDS exchange instead of following it.
 // Off for padded lengths above 4096 (four passes), where the extra registers
 // spill at 1024 threads.
+#ifndef MDFT_TW_PREFETCH_MAXLOGM
+#define MDFT_TW_PREFETCH_MAXLOGM 12
+#endif
 template <int LOGM>
 __host__ __device__ constexpr bool tw_prefetch()
 {
-    return MDFT_TW_PREFETCH && LOGM <= 12;
+    return MDFT_TW_PREFETCH && LOGM <= MDFT_TW_PREFETCH_MAXLOGM;
+}
+
+#ifndef MDFT_TW_LOADED
+#define MDFT_TW_LOADED 4
+#endif
+// Twiddles of a radix-R row read from the table (w^0 .. w^(L-1)); the others
+// are products of those (w^(aL+b) = w^(aL) * w^b), at most four roundings
+// from the FP64-generated entries.
+template <int R>
+__host__ __device__ constexpr int tw_loaded()
+{
+    return R < MDFT_TW_LOADED ? R : MDFT_TW_LOADED;
 }
 
 template <int LOGM, int THREADS, int P>
 struct TwLoad {
     using C = LineCfg<LOGM, THREADS>;
     static constexpr int R = 1 << pass_bits(LOGM, P);
+    static constexpr int L = tw_loaded<R>();
     static constexpr int NB = C::EPT / R;
     static constexpr int NS = 1 << (4 * P);
     __device__ __forceinline__ static void run(float4 (&w)[C::EPT / 2], int t, const float2* tw)
@@ -370,10 +386,30 @@ struct TwLoad {
             const int k = (t + b * C::TPL) & (NS - 1);
             const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * R);
 #pragma unroll
-            for (int r2 = 0; r2 < R / 2; ++r2) w[b * (R / 2) + r2] = row[r2];
+            for (int r2 = 0; r2 < L / 2; ++r2) w[b * (L / 2) + r2] = row[r2];
         }
     }
 };
+
+// the R twiddles of one butterfly from its L loaded ones (L/2 float4)
+template <int R, int L>
+__device__ __forceinline__ void expand_twiddles(float2 (&wt)[R], const float4* q)
+{
+#pragma unroll
+    for (int r2 = 0; r2 < L / 2; ++r2) {
+        wt[2 * r2] = make_float2(q[r2].x, q[r2].y);
+        wt[2 * r2 + 1] = make_float2(q[r2].z, q[r2].w);
+    }
+    if constexpr (L < R) {
+        wt[L] = cmul(wt[L / 2], wt[L / 2]);
+#pragma unroll
+        for (int a = 2; a * L < R; ++a) wt[a * L] = cmul(wt[(a - 1) * L], wt[L]);
+#pragma unroll
+        for (int a = 1; a * L < R; ++a)
+#pragma unroll
+            for (int b = 1; b < L; ++b) wt[a * L + b] = cmul(wt[a * L], wt[b]);
+    }
+}
 
 template <int LOGM, int THREADS, bool COLS, int P>
 struct FftPass {
@@ -410,13 +446,15 @@ struct FftPass {
 #pragma unroll
             for (int r = 0; r < R; ++r) u[r] = v[b + NB * r];
             if constexpr (P > 0) {
+                constexpr int L = tw_loaded<R>();
+                float4 q[L / 2];
                 const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * R);
 #pragma unroll
-                for (int r2 = 0; r2 < R / 2; ++r2) {
-                    const float4 q = tw_prefetch<LOGM>() ? w[b * (R / 2) + r2] : row[r2];
-                    if (r2 > 0) u[2 * r2] = cmul(u[2 * r2], make_float2(q.x, q.y));
-                    u[2 * r2 + 1] = cmul(u[2 * r2 + 1], make_float2(q.z, q.w));
-                }
+                for (int r2 = 0; r2 < L / 2; ++r2) q[r2] = tw_prefetch<LOGM>() ? w[b * (L / 2) + r2] : row[r2];
+                float2 wt[R];
+                expand_twiddles<R, L>(wt, q);
+#pragma unroll
+                for (int r = 1; r < R; ++r) u[r] = cmul(u[r], wt[r]);
             }
             dft_fwd<R>(u);
 #pragma unroll

@@ -182,7 +182,7 @@ def main():
             "traffic": measured_traffic(),
             "traffic_source": "profiles/batch_mixed256_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE per step, "
                               "scaled by a 1 GiB copy)",
-            "kernel": "k_batch_grid<256> (whole small images + row tiles) then k_batch_grid<1024> (column tiles)",
+            "kernel": "k_batch_grid<256> (whole small images + row tiles) then k_batch_grid<512> (column tiles)",
             "alg_bytes_per_launch": alg_bytes,
             "launch_ms": round(kernel_ms, 4),
         },

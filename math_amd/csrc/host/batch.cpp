@@ -90,7 +90,7 @@ int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count,
     b->threads = T;
     // grid launches run column tiles in wide tiles; the persistent shapes use one size
     const bool grid_shape = !(flags & (MDFT_BATCH_PERSISTENT | MDFT_BATCH_DEPENDENT));
-    b->threads_cols = grid_shape ? 1024 : T;
+    b->threads_cols = grid_shape ? (mdft::BATCH_COL_THREADS > T ? mdft::BATCH_COL_THREADS : T) : T;
     b->descs.assign(descs, descs + count);
     std::vector<mdft::BatchEntry> entries(count);
     for (uint32_t i = 0; i < count; ++i) {

@@ -19,6 +19,8 @@
 
 namespace mdft {
 
+void arrange_for_xcds(const std::vector<uint2>& items, const std::vector<double>& cost, std::vector<uint2>& out);
+
 int batch_threads_for(const uint32_t* widths, const uint32_t* heights, size_t count)
 {
     const int limit = batch_max_logm(1024);
@@ -141,12 +143,79 @@ void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, i
             return std::make_tuple(a.phase, -a.logm, a.blue, a.kind, a.seq) <
                    std::make_tuple(c.phase, -c.logm, c.blue, c.kind, c.seq);
         });
+        std::vector<uint2> p1, p2;
+        std::vector<double> c1, c2;
         for (const Item& it : items) {
-            if (it.phase == 1) ++out.nphase1;
-            out.items.push_back(it.v);
+            // cost of a tile ~ points x passes (x2 for Bluestein's two FFTs)
+            const int lpw = batch_lines_per_tile(it.kind == 2 && it.phase == 2 ? TC : T, it.logm);
+            const double cost = double(lpw) * double(1 << it.logm) * (it.logm + 2) * (it.blue ? 2.0 : 1.0) *
+                                (it.kind == 0 ? 2.0 : 1.0);
+            (it.phase == 1 ? p1 : p2).push_back(it.v);
+            (it.phase == 1 ? c1 : c2).push_back(cost);
         }
+        out.nphase1 = (uint32_t)p1.size();
+        arrange_for_xcds(p1, c1, out.items);
+        arrange_for_xcds(p2, c2, out.items);
     }
     out.ncounters = counter;
+}
+
+// Workgroups of a launch are dealt to the XCDs round-robin: block b runs on
+// XCD b % 8, and each XCD has its own L2.  Give every job (its tiles, and
+// with them its chirp, diagonal and twiddle tables and, for column tiles,
+// the cache lines neighbouring tiles share) to one XCD, balancing the XCDs'
+// total cost (largest job first, jobs larger than an eighth of the launch
+// split), then lay the eight queues out so that queue x fills the blocks
+// b % 8 == x.  Within a queue the variant order of `items` is kept.
+void arrange_for_xcds(const std::vector<uint2>& items, const std::vector<double>& cost, std::vector<uint2>& out)
+{
+    constexpr int NXCD = 8;
+    const size_t n = items.size();
+    if (n == 0) return;
+    double total = 0;
+    for (double c : cost) total += c;
+    // runs of consecutive items of one job (items are grouped by variant, then job)
+    struct Run {
+        size_t begin, end;
+        double cost;
+    };
+    std::vector<Run> runs;
+    const double cap = total / (2 * NXCD);
+    for (size_t i = 0; i < n;) {
+        size_t j = i;
+        double c = 0;
+        while (j < n && items[j].x == items[i].x && (j == i || c + cost[j] <= cap)) c += cost[j++];
+        runs.push_back({i, j, c});
+        i = j;
+    }
+    std::vector<size_t> by_cost(runs.size());
+    std::iota(by_cost.begin(), by_cost.end(), size_t(0));
+    std::stable_sort(by_cost.begin(), by_cost.end(), [&](size_t a, size_t b) { return runs[a].cost > runs[b].cost; });
+    std::vector<int> owner(runs.size());
+    double load[NXCD] = {};
+    for (size_t r : by_cost) {
+        int best = 0;
+        for (int x = 1; x < NXCD; ++x)
+            if (load[x] < load[best]) best = x;
+        owner[r] = best;
+        load[best] += runs[r].cost;
+    }
+    std::vector<uint2> queue[NXCD];
+    for (size_t r = 0; r < runs.size(); ++r)
+        for (size_t i = runs[r].begin; i < runs[r].end; ++i) queue[owner[r]].push_back(items[i]);
+    // interleave; when a queue runs dry its slots take the longest remaining queue's items
+    size_t pos[NXCD] = {};
+    for (size_t b = 0; b < n; ++b) {
+        int x = int(b % NXCD);  // block index within this launch
+        if (pos[x] == queue[x].size()) {
+            int best = -1;
+            for (int y = 0; y < NXCD; ++y)
+                if (pos[y] < queue[y].size() && (best < 0 || queue[y].size() - pos[y] > queue[best].size() - pos[best]))
+                    best = y;
+            x = best;
+        }
+        out.push_back(queue[x][pos[x]++]);
+    }
 }
 
 }  // namespace mdft

@@ -173,7 +173,7 @@ void launch_micro_batch(mdft_server_s* s, Lane& L, const std::vector<mdft::Batch
     }
     const int T = mdft::batch_threads_for(ws.data(), hs.data(), entries.size());
     mdft::BatchBuild bb;
-    mdft::build_batch(ctx, entries.data(), entries.size(), T, 1024, false, bb);
+    mdft::build_batch(ctx, entries.data(), entries.size(), T, std::max(T, mdft::BATCH_COL_THREADS), false, bb);
     grow(L, bb.jobs.size(), bb.items.size());
     std::copy(bb.jobs.begin(), bb.jobs.end(), L.jobs_host);
     std::copy(bb.items.begin(), bb.items.end(), L.items_host);

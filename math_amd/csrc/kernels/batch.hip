@@ -6,11 +6,11 @@
 // own waves, then runs the column tile, with no cross-workgroup
 // synchronisation at all.  Three launch shapes share the item code:
 //
-//  * k_batch_grid: one workgroup per item (the hardware dispatcher balances
-//    the load); a batch is two launches, whole images + row tiles (256-thread
-//    tiles), then column tiles (1024-thread tiles, 4-16 columns each for
-//    padded lengths 1024-4096, so each row of a tile is a longer contiguous
-//    run).
+//  * k_batch_grid: one workgroup per item, items laid out by the host so
+//    each job runs on one XCD; a batch is two launches, whole images + row
+//    tiles (256-thread tiles), then column tiles (512-thread tiles, 2-8
+//    columns each for padded lengths 1024-4096, so each row of a tile is a
+//    longer contiguous run, with two workgroups per CU).
 //  * k_batch_persistent: a grid sized to the device's residency takes items
 //    from a ticket counter, one launch per phase.
 //  * k_batch_persistent with dependencies: one launch for everything; a
@@ -30,18 +30,6 @@
 namespace mdft {
 
 __host__ __device__ constexpr int batch_maxlogm(int threads) { return threads >= 512 ? 13 : 12; }
-
-// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
-// Within each run of 64 blocks, give XCD x the 8 consecutive items
-// 8x..8x+7, so neighbouring tiles of one job (columns sharing cache lines)
-// run at the same time in the same L2.  A partial last run keeps its order.
-__device__ __forceinline__ uint32_t xcd_group_perm(uint32_t p, uint32_t n)
-{
-    const uint32_t g = p >> 6;
-    if ((g + 1) * 64 > n) return p;
-    const uint32_t q = p & 63;
-    return (g << 6) + ((q & 7) << 3) + (q >> 3);
-}
 
 template <int LOGM, int T>
 constexpr int lds_need()
@@ -152,13 +140,14 @@ __device__ __forceinline__ void run_item(const BatchJob* __restrict__ jobs, uint
 }
 
 // Four waves per SIMD (<= 128 VGPRs): 256-thread tiles (4 workgroups per
-// CU) or 1024-thread tiles (one, with 4x the lines per tile).
+// CU), 512-thread tiles (two) or 1024-thread tiles (one).
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_grid(
     const BatchJob* __restrict__ jobs, const uint2* __restrict__ items)
 {
     __shared__ float lds[batch_lds<T>()];
-    run_item<T, true>(jobs, items[xcd_group_perm(blockIdx.x, gridDim.x)], lds);
+    // the host lays items out per XCD (block b runs on XCD b % 8)
+    run_item<T, true>(jobs, items[blockIdx.x], lds);
 }
 
 // The item loop contains barriers, so its control flow is kept free of
@@ -228,6 +217,8 @@ hipError_t launch_batch_grid(int threads, const BatchJob* jobs, const uint2* ite
     if (nitems == 0) return hipSuccess;
     if (threads == 256)
         hipLaunchKernelGGL(k_batch_grid<256>, dim3(nitems), dim3(256), 0, stream, jobs, items);
+    else if (threads == 512)
+        hipLaunchKernelGGL(k_batch_grid<512>, dim3(nitems), dim3(512), 0, stream, jobs, items);
     else
         hipLaunchKernelGGL(k_batch_grid<1024>, dim3(nitems), dim3(1024), 0, stream, jobs, items);
     return hipGetLastError();

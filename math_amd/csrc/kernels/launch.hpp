@@ -70,7 +70,11 @@ hipError_t launch_image_axis(int logm, const AxisJob& job, const ImageIO& io, bo
 hipError_t launch_image_luminance(const ImageIO& io, float2* dst, int w, int h, hipStream_t stream);
 hipError_t launch_image_magnitude(const ImageIO& io, const float2* src, int w, int h, hipStream_t stream);
 
-// batch kernels: threads is 256 (padded lengths up to 4096) or 1024 (up to 8192)
+// Workgroup size of batch column tiles: 2-16 columns per tile for padded
+// lengths 1024-4096 (wider contiguous runs per row), two workgroups per CU.
+constexpr int BATCH_COL_THREADS = 512;
+
+// batch kernels: threads is 256 (padded lengths up to 4096), 512 or 1024 (up to 8192)
 hipError_t launch_batch_grid(int threads, const BatchJob* jobs, const uint2* items, uint32_t nitems,
                              hipStream_t stream);
 hipError_t launch_batch_persistent(int threads, const BatchJob* jobs, const uint2* items, uint32_t nitems,

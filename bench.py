@@ -194,6 +194,7 @@ def main():
         out["plain_path"]["speedup_batched_vs_device_plain"] = round(
             value / out["plain_path"]["device_gflops"], 2)
         out["plain_path"]["speedup_batched_vs_host_plain"] = round(value / out["plain_path"]["host_gflops"], 2)
+        out["single"] = single_transforms(M, torch)
 
     if args.calibrate_copy:
         a = torch.empty(1 << 27, dtype=torch.complex64, device=dev)
@@ -234,6 +235,37 @@ def accuracy_sample(M, torch, src, dst, sizes, invs, offs, count=12):
         worst["rel_l1"] = max(worst["rel_l1"], m["rel_l1"])
         worst["tol_ratio"] = max(worst["tol_ratio"], m["nl2"] / tol)
     return {k: float("%.3e" % v) for k, v in worst.items()} | {"sampled": len(pick), "oracle": "fp64"}
+
+
+def single_transforms(M, torch, reps=20):
+    """One image per call on device memory (mdft_exec_device), event-timed: the
+    single-image configurations 640x480 (forward and inverse), 1024x1024 (round
+    trip) and 4093x2047, with their algorithmic bandwidth 32*w*h bytes per
+    transform."""
+    res = {}
+    sh = torch.cuda.current_stream().cuda_stream
+    g = torch.Generator(device="cuda:0")
+    for (w, h, invs) in [(640, 480, (False, True)), (1024, 1024, (False, True)), (4093, 2047, (False,))]:
+        g.manual_seed(SEED_BASE + w)
+        x = torch.randn(w * h, dtype=torch.complex64, device="cuda:0", generator=g)
+        y = torch.empty_like(x)
+        p = M.Plan(w, h)
+        for inv in invs:
+            for _ in range(3):
+                p.exec_device(inv, x.data_ptr(), y.data_ptr(), sh)
+            s0 = torch.cuda.Event(enable_timing=True)
+            s1 = torch.cuda.Event(enable_timing=True)
+            s0.record()
+            for _ in range(reps):
+                p.exec_device(inv, x.data_ptr(), y.data_ptr(), sh)
+            s1.record()
+            torch.cuda.synchronize()
+            us = s0.elapsed_time(s1) * 1e3 / reps
+            res[f"{w}x{h}_{'inv' if inv else 'fwd'}"] = {"us": round(us, 1),
+                                                          "alg_gbs": round(32.0 * w * h / (us * 1e-6) / 1e9, 1)}
+        p.close()
+        del x, y
+    return res
 
 
 def plain_paths(M, torch, src, dst, sizes, invs, offs, flops):

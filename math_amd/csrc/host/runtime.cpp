@@ -274,8 +274,18 @@ void Context::enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, i
         return;
     }
     const AxisJob j = make_job(lp, src, dst, es, ls, count, inverse);
-    hip_check(launch_axis(lp.logm, j, lp.blue, cols, variant(lp.logm, lp.blue, cols) != 0, stream),
+    hip_check(launch_axis(lp.logm, j, lp.blue, cols, wide_tiles(lp.logm, lp.blue, cols, count), stream),
               "axis pass launch");
+}
+
+bool Context::wide_tiles(int logm, bool blue, bool cols, int count) const
+{
+    if (variant(logm, blue, cols) == 0) return false;
+    // wide tiles hold 4x the lines of narrow ones; keep at least one tile per CU
+    const int tpl = (1 << logm) / ept_of(logm);
+    const int lines_wide = tpl >= 1024 ? 1 : 1024 / tpl;
+    const int64_t tiles = (int64_t(count) + lines_wide - 1) / lines_wide;
+    return tiles >= int64_t(cu_count_);
 }
 
 void Context::enqueue_transform(const LinePlan& row, const LinePlan& col, int w, int h, bool inverse,

@@ -90,6 +90,8 @@ public:
     // Tile-shape table of the single-image kernels: variant 0 = narrow
     // (256-thread) tiles, 1 = wide (1024-thread) tiles, per (logm, blue, cols).
     int variant(int logm, bool blue, bool cols) const { return variant_[logm][blue][cols]; }
+    // the table's choice, narrowed when wide tiles would leave CUs without work
+    bool wide_tiles(int logm, bool blue, bool cols, int count) const;
     void set_variant(int logm, bool blue, bool cols, int v) { variant_[logm][blue][cols] = (uint8_t)v; }
 
 private:

@@ -121,8 +121,11 @@ def main():
     src = torch.randn(total, dtype=torch.complex64, device=dev, generator=g)
     dst = torch.empty_like(src)
 
-    flags = {"graph": M.MDFT_BATCH_GRAPH, "direct": 0, "persistent": M.MDFT_BATCH_PERSISTENT,
-             "dependent": M.MDFT_BATCH_DEPENDENT}[args.mode]
+    # autotuning (outside the timed region): tile shapes of the single-image
+    # kernels, and the batch's column-tile size measured on scratch images
+    tune_s = M.tune_auto()
+    flags = {"graph": M.MDFT_BATCH_GRAPH | M.MDFT_BATCH_TUNE, "direct": M.MDFT_BATCH_TUNE,
+             "persistent": M.MDFT_BATCH_PERSISTENT, "dependent": M.MDFT_BATCH_DEPENDENT}[args.mode]
     batch = M.Batch(sizes, invs, offs, offs, flags)
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -169,6 +172,7 @@ def main():
             "points": total,
             "seed": SEED_BASE + 3,
             "launch": args.mode,
+            "autotune_s": round(tune_s, 2),
         },
         "gbs": round(alg_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "p50_ms": round(float(np.percentile(step_ms, 50)), 4),

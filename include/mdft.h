@@ -98,6 +98,19 @@ int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, vo
 int mdft_tune_get(int logm, int blue, int cols, int* variant);
 int mdft_tune_set(int logm, int blue, int cols, int variant);
 
+/* Autotuner: time both tile shapes of every (padded length, Bluestein, axis)
+ * kernel on this thread's device (64 MiB of lines each, in a private buffer)
+ * and keep the faster one in the table; *seconds (optional) receives the
+ * time taken.  When the wide shape would leave CUs without a tile, the
+ * narrow one runs regardless of the table. */
+int mdft_tune_auto(double* seconds);
+
+/* Save / load the table as text, one "logm blue cols variant" line per entry
+ * ('#' starts a comment).  A file with a bad entry is rejected as a whole
+ * (MDFT_ERR_INVALID). */
+int mdft_tune_save(const char* path);
+int mdft_tune_load(const char* path);
+
 /* ---- batched power-of-two 1-D FFT ------------------------------------------ */
 
 typedef struct mdft_fft_s* mdft_fft;
@@ -177,6 +190,8 @@ typedef struct mdft_batch_s* mdft_batch;
 #define MDFT_BATCH_GRAPH 1u       /* replay the launches as a captured hipGraph */
 #define MDFT_BATCH_PERSISTENT 2u  /* persistent workgroups taking items from a ticket counter */
 #define MDFT_BATCH_DEPENDENT 4u   /* one persistent launch; column tiles wait on per-image row counters */
+#define MDFT_BATCH_TUNE 8u        /* grid shape: pick the column-tile size (256/512/1024 threads) by timing
+                                     this batch's column phase on scratch images at create time */
 
 /* Plan a batch of transforms (no counterpart in the reference, whose exec
  * handles one image per call).  Every side must have a padded length

@@ -35,6 +35,7 @@ SOURCES = [
     "host/capi.cpp",
     "host/fft1d.cpp",
     "host/image.cpp",
+    "host/tune.cpp",
     "host/batch.cpp",
     "host/batch_builder.cpp",
     "host/server.cpp",

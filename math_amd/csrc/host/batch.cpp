@@ -65,6 +65,78 @@ void enqueue_batch(mdft_batch_s* b, hipStream_t stream)
     }
 }
 
+// Time the column phase of one batch layout (column tiles of TC threads) on
+// zeroed scratch images: build, upload, one warm-up and three timed launches.
+float time_column_phase(Context& ctx, const std::vector<mdft::BatchEntry>& entries, int T, int TC, hipStream_t s)
+{
+    mdft::BatchBuild bb;
+    mdft::build_batch(ctx, entries.data(), entries.size(), T, TC, false, bb);
+    const uint32_t n2 = (uint32_t)bb.items.size() - bb.nphase1;
+    if (n2 == 0) return 0.0f;
+    BatchJob* jobs = nullptr;
+    uint2* items = nullptr;
+    hipEvent_t a = nullptr, e = nullptr;
+    float ms = 0.0f;
+    hipError_t err = hipMalloc(&jobs, sizeof(BatchJob) * bb.jobs.size());
+    if (err == hipSuccess) err = hipMalloc(&items, sizeof(uint2) * n2);
+    if (err == hipSuccess) err = hipMemcpy(jobs, bb.jobs.data(), sizeof(BatchJob) * bb.jobs.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(items, bb.items.data() + bb.nphase1, sizeof(uint2) * n2, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipEventCreate(&a);
+    if (err == hipSuccess) err = hipEventCreate(&e);
+    if (err == hipSuccess) err = mdft::launch_batch_grid(TC, jobs, items, n2, s);
+    if (err == hipSuccess) err = hipEventRecord(a, s);
+    for (int r = 0; r < 3 && err == hipSuccess; ++r) err = mdft::launch_batch_grid(TC, jobs, items, n2, s);
+    if (err == hipSuccess) err = hipEventRecord(e, s);
+    if (err == hipSuccess) err = hipEventSynchronize(e);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, e);
+    (void)hipStreamSynchronize(s);
+    if (a) (void)hipEventDestroy(a);
+    if (e) (void)hipEventDestroy(e);
+    if (jobs) (void)hipFree(jobs);
+    if (items) (void)hipFree(items);
+    mdft::hip_check(err, "batch tuning");
+    return ms / 3.0f;
+}
+
+// Column-tile size of a grid-shape batch picked by measurement: 256 (when the
+// row tiles are 256-thread), 512 or 1024 threads.
+int tune_column_threads(Context& ctx, const mdft_desc* descs, uint32_t count, int T)
+{
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < count; ++i) total += (uint64_t)descs[i].width * descs[i].height;
+    float2* scratch = nullptr;
+    mdft::hip_check(hipMalloc(&scratch, sizeof(float2) * std::max<uint64_t>(total, 1)), "hipMalloc tuning images");
+    hipStream_t s = nullptr;
+    int best = mdft::BATCH_COL_THREADS > T ? mdft::BATCH_COL_THREADS : T;
+    try {
+        mdft::hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        mdft::hip_check(hipMemsetAsync(scratch, 0, sizeof(float2) * std::max<uint64_t>(total, 1), s), "clear");
+        std::vector<mdft::BatchEntry> entries(count);
+        uint64_t off = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            entries[i] = {descs[i].width, descs[i].height, descs[i].inverse != 0, scratch + off, scratch + off};
+            off += (uint64_t)descs[i].width * descs[i].height;
+        }
+        float best_ms = -1.0f;
+        for (int tc : {256, 512, 1024}) {
+            if (tc < T) continue;
+            const float ms = time_column_phase(ctx, entries, T, tc, s);
+            if (best_ms < 0 || ms < best_ms) {
+                best_ms = ms;
+                best = tc;
+            }
+        }
+    } catch (...) {
+        if (s) (void)hipStreamDestroy(s);
+        (void)hipFree(scratch);
+        throw;
+    }
+    (void)hipStreamDestroy(s);
+    (void)hipFree(scratch);
+    return best;
+}
+
 }  // namespace
 
 extern "C" {
@@ -91,6 +163,7 @@ int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count,
     // grid launches run column tiles in wide tiles; the persistent shapes use one size
     const bool grid_shape = !(flags & (MDFT_BATCH_PERSISTENT | MDFT_BATCH_DEPENDENT));
     b->threads_cols = grid_shape ? (mdft::BATCH_COL_THREADS > T ? mdft::BATCH_COL_THREADS : T) : T;
+    if (grid_shape && (flags & MDFT_BATCH_TUNE)) b->threads_cols = tune_column_threads(ctx, descs, count, T);
     b->descs.assign(descs, descs + count);
     std::vector<mdft::BatchEntry> entries(count);
     for (uint32_t i = 0; i < count; ++i) {

@@ -33,6 +33,7 @@ MDFT_ERR_STATE = -8
 MDFT_BATCH_GRAPH = 1
 MDFT_BATCH_PERSISTENT = 2
 MDFT_BATCH_DEPENDENT = 4
+MDFT_BATCH_TUNE = 8
 
 
 class ErrorException(RuntimeError):
@@ -104,6 +105,9 @@ def lib():
             "mdft_exec_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
             "mdft_tune_get": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
             "mdft_tune_set": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+            "mdft_tune_auto": ([ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+            "mdft_tune_save": ([ctypes.c_char_p], ctypes.c_int),
+            "mdft_tune_load": ([ctypes.c_char_p], ctypes.c_int),
             "mdft_fft_create": ([ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int),
             "mdft_fft_exec": ([_vp, ctypes.c_int, _vp, ctypes.c_size_t], ctypes.c_int),
             "mdft_fft_exec_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
@@ -158,6 +162,22 @@ def tune_get(logm, blue, cols):
 
 def tune_set(logm, blue, cols, variant):
     _check(lib().mdft_tune_set(logm, int(blue), int(cols), int(variant)))
+
+
+def tune_auto():
+    """Time both tile shapes of every single-image kernel and keep the faster
+    (mdft_tune_auto); returns the seconds taken."""
+    sec = ctypes.c_double()
+    _check(lib().mdft_tune_auto(ctypes.byref(sec)))
+    return sec.value
+
+
+def tune_save(path):
+    _check(lib().mdft_tune_save(os.fsencode(path)))
+
+
+def tune_load(path):
+    _check(lib().mdft_tune_load(os.fsencode(path)))
 
 
 class Plan:

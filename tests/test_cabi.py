@@ -96,3 +96,12 @@ def test_image_create_rejects_bad_format_and_size():
     assert M.lib().mdft_image_create(ctypes.byref(h), 0, 8, 0) == M.MDFT_ERR_SIZE
     assert M.lib().mdft_last_error() == b"Error size 0 for compute m"
     assert M.lib().mdft_image_destroy(None) == M.MDFT_OK
+
+
+def test_tune_load_rejects_bad_files(tmp_path):
+    p = tmp_path / "bad.txt"
+    p.write_text("# comment\n3 0 1 1\n99 0 0 1\n")
+    with pytest.raises(M.ErrorException, match="bad entry on line 3"):
+        M.tune_load(str(p))
+    with pytest.raises(M.ErrorException, match="cannot read"):
+        M.tune_load(str(tmp_path / "missing.txt"))

@@ -67,7 +67,8 @@ def single(M, x, inv):
     return d
 
 
-@pytest.mark.parametrize("flags,in_place", [(0, False), (0, True), (1, False), (2, False), (3, True), (4, False), (5, True)])
+@pytest.mark.parametrize("flags,in_place", [(0, False), (0, True), (1, False), (2, False), (3, True), (4, False), (5, True),
+                                            (9, False), (8, True)])
 def test_batch_matches_single_and_oracle(env, flags, in_place):
     M, torch = env
     rng = np.random.default_rng(100 + flags)

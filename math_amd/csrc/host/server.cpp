@@ -113,10 +113,9 @@ struct Lane {
     hipEvent_t done = nullptr;
     bool busy = false;
     std::vector<uint64_t> reqs;
-    BatchJob* jobs_dev = nullptr;
-    uint2* items_dev = nullptr;
-    BatchJob* jobs_host = nullptr;  // pinned staging
-    uint2* items_host = nullptr;
+    // one device table and its pinned staging copy: cap_jobs jobs, then items
+    char* table_dev = nullptr;
+    char* table_host = nullptr;
     size_t cap_jobs = 0, cap_items = 0;
 };
 
@@ -142,24 +141,19 @@ struct mdft_server_s {
 
 namespace {
 
+size_t table_bytes(size_t jobs, size_t items) { return sizeof(BatchJob) * jobs + sizeof(uint2) * items; }
+
 void grow(Lane& L, size_t njobs, size_t nitems)
 {
-    if (njobs > L.cap_jobs) {
-        if (L.jobs_dev) mdft::hip_check(hipFree(L.jobs_dev), "hipFree");
-        if (L.jobs_host) mdft::hip_check(hipHostFree(L.jobs_host), "hipHostFree");
-        const size_t cap = std::max<size_t>(njobs, 2 * L.cap_jobs);
-        mdft::hip_check(hipMalloc(&L.jobs_dev, sizeof(BatchJob) * cap), "hipMalloc");
-        mdft::hip_check(hipHostMalloc(&L.jobs_host, sizeof(BatchJob) * cap, hipHostMallocDefault), "hipHostMalloc");
-        L.cap_jobs = cap;
-    }
-    if (nitems > L.cap_items) {
-        if (L.items_dev) mdft::hip_check(hipFree(L.items_dev), "hipFree");
-        if (L.items_host) mdft::hip_check(hipHostFree(L.items_host), "hipHostFree");
-        const size_t cap = std::max<size_t>(nitems, 2 * L.cap_items);
-        mdft::hip_check(hipMalloc(&L.items_dev, sizeof(uint2) * cap), "hipMalloc");
-        mdft::hip_check(hipHostMalloc(&L.items_host, sizeof(uint2) * cap, hipHostMallocDefault), "hipHostMalloc");
-        L.cap_items = cap;
-    }
+    if (njobs <= L.cap_jobs && nitems <= L.cap_items) return;
+    if (L.table_dev) mdft::hip_check(hipFree(L.table_dev), "hipFree");
+    if (L.table_host) mdft::hip_check(hipHostFree(L.table_host), "hipHostFree");
+    L.table_dev = L.table_host = nullptr;
+    const size_t cj = std::max(njobs, 2 * L.cap_jobs), ci = std::max(nitems, 2 * L.cap_items);
+    mdft::hip_check(hipMalloc(&L.table_dev, table_bytes(cj, ci)), "hipMalloc");
+    mdft::hip_check(hipHostMalloc(&L.table_host, table_bytes(cj, ci), hipHostMallocDefault), "hipHostMalloc");
+    L.cap_jobs = cj;
+    L.cap_items = ci;
 }
 
 // Launch one micro-batch on lane L (scheduler thread, lock not held).
@@ -175,23 +169,26 @@ void launch_micro_batch(mdft_server_s* s, Lane& L, const std::vector<mdft::Batch
     mdft::BatchBuild bb;
     mdft::build_batch(ctx, entries.data(), entries.size(), T, std::max(T, mdft::BATCH_COL_THREADS), false, bb);
     grow(L, bb.jobs.size(), bb.items.size());
-    std::copy(bb.jobs.begin(), bb.jobs.end(), L.jobs_host);
-    std::copy(bb.items.begin(), bb.items.end(), L.items_host);
-    if (!bb.jobs.empty())
-        mdft::hip_check(hipMemcpyAsync(L.jobs_dev, L.jobs_host, sizeof(BatchJob) * bb.jobs.size(),
-                                       hipMemcpyHostToDevice, L.stream),
-                        "upload jobs");
-    if (!bb.items.empty())
-        mdft::hip_check(hipMemcpyAsync(L.items_dev, L.items_host, sizeof(uint2) * bb.items.size(),
-                                       hipMemcpyHostToDevice, L.stream),
-                        "upload items");
+    // jobs at the start of the table, items after the lane's job capacity: one upload
+    BatchJob* jobs_host = reinterpret_cast<BatchJob*>(L.table_host);
+    uint2* items_host = reinterpret_cast<uint2*>(L.table_host + sizeof(BatchJob) * L.cap_jobs);
+    std::copy(bb.jobs.begin(), bb.jobs.end(), jobs_host);
+    std::copy(bb.items.begin(), bb.items.end(), items_host);
+    const BatchJob* jobs_dev = reinterpret_cast<const BatchJob*>(L.table_dev);
+    const uint2* items_dev = reinterpret_cast<const uint2*>(L.table_dev + sizeof(BatchJob) * L.cap_jobs);
+    if (!bb.items.empty()) {
+        // the used part: [0, jobs) and [cap_jobs, cap_jobs + items) as one span
+        const size_t span = sizeof(BatchJob) * L.cap_jobs + sizeof(uint2) * bb.items.size();
+        mdft::hip_check(hipMemcpyAsync(L.table_dev, L.table_host, span, hipMemcpyHostToDevice, L.stream),
+                        "upload tables");
+    }
     // 1x1 requests are identities: copy the single value
     for (const auto& e : entries)
         if (e.width == 1 && e.height == 1)
             mdft::hip_check(hipMemcpyAsync(e.dst, e.src, sizeof(float2), hipMemcpyDeviceToDevice, L.stream), "copy");
     const uint32_t n = (uint32_t)bb.items.size();
-    mdft::hip_check(mdft::launch_batch_grid(T, L.jobs_dev, L.items_dev, bb.nphase1, L.stream), "serve launch");
-    mdft::hip_check(mdft::launch_batch_grid(bb.threads_cols, L.jobs_dev, L.items_dev + bb.nphase1, n - bb.nphase1,
+    mdft::hip_check(mdft::launch_batch_grid(T, jobs_dev, items_dev, bb.nphase1, L.stream), "serve launch");
+    mdft::hip_check(mdft::launch_batch_grid(bb.threads_cols, jobs_dev, items_dev + bb.nphase1, n - bb.nphase1,
                                             L.stream),
                     "serve launch");
     mdft::hip_check(hipEventRecord(L.done, L.stream), "event record");
@@ -303,7 +300,9 @@ int mdft_server_create(mdft_server* server, uint64_t hbm_budget_bytes, uint32_t 
     for (Lane& L : s->lanes) {
         mdft::hip_check(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking), "hipStreamCreate");
         mdft::hip_check(hipEventCreateWithFlags(&L.done, hipEventDisableTiming), "hipEventCreate");
-        grow(L, 64, 1024);
+        // sized up front for max_batch requests of up to 16384 tiles each, so
+        // no launch has to reallocate (a hipFree would wait for the whole device)
+        grow(L, 2 * (size_t)s->max_batch, std::max<size_t>(size_t(1) << 20, 16384 * (size_t)s->max_batch / 4));
     }
     s->sched = std::thread(scheduler, s.get());
     *server = s.release();
@@ -478,10 +477,8 @@ int mdft_server_destroy(mdft_server s)
                 (void)hipStreamDestroy(L.stream);
             }
             if (L.done) (void)hipEventDestroy(L.done);
-            if (L.jobs_dev) (void)hipFree(L.jobs_dev);
-            if (L.items_dev) (void)hipFree(L.items_dev);
-            if (L.jobs_host) (void)hipHostFree(L.jobs_host);
-            if (L.items_host) (void)hipHostFree(L.items_host);
+            if (L.table_dev) (void)hipFree(L.table_dev);
+            if (L.table_host) (void)hipHostFree(L.table_host);
         }
         if (s->arena_mem) (void)hipFree(s->arena_mem);
     }

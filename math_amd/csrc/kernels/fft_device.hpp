@@ -66,11 +66,14 @@ __host__ __device__ constexpr int ept_of(int logm) { return logm < 4 ? (1 << log
 __host__ __device__ constexpr int npass_of(int logm) { return (logm + 3) / 4; }
 __host__ __device__ constexpr int pass_bits(int logm, int p) { return (logm - 4 * p) >= 4 ? 4 : (logm - 4 * p); }
 
-// Twiddle rows of pass p >= 1 (Ns = 16^p before the pass, radix R):
-// row k (k < Ns) holds R entries exp(-2*pi*i*k*r/(Ns*R)), r < R.
+// Twiddle rows of pass p >= 1 (Ns = 16^p before the pass, radix R): row k
+// (k < Ns) holds TW_ROW = 4 entries w^1, w^2, w^3, w^4 of
+// w = exp(-2*pi*i*k/(Ns*R)) (32 bytes, four rows per cache line); the other
+// powers are products of these (expand_twiddles).
+constexpr int TW_ROW = 4;
 __host__ __device__ constexpr int64_t tw_pass_len(int logm, int p)
 {
-    return (int64_t(1) << (4 * p)) * (int64_t(1) << pass_bits(logm, p));
+    return (pass_bits(logm, p) > 0 ? (int64_t(1) << (4 * p)) : 0) * TW_ROW;
 }
 __host__ __device__ constexpr int64_t tw_pass_off(int logm, int p)
 {
@@ -360,54 +363,57 @@ __host__ __device__ constexpr bool tw_prefetch()
     return MDFT_TW_PREFETCH && LOGM <= MDFT_TW_PREFETCH_MAXLOGM;
 }
 
-#ifndef MDFT_TW_LOADED
-#define MDFT_TW_LOADED 4
-#endif
-// Twiddles of a radix-R row read from the table (w^0 .. w^(L-1)); the others
-// are products of those (w^(aL+b) = w^(aL) * w^b), at most four roundings
-// from the FP64-generated entries.
+// float4s of a row a radix-R butterfly reads: (w^1, w^2) and (w^3, w^4)
 template <int R>
-__host__ __device__ constexpr int tw_loaded()
+__host__ __device__ constexpr int tw_quads()
 {
-    return R < MDFT_TW_LOADED ? R : MDFT_TW_LOADED;
+    return R <= 2 ? 1 : 2;
 }
 
 template <int LOGM, int THREADS, int P>
 struct TwLoad {
     using C = LineCfg<LOGM, THREADS>;
     static constexpr int R = 1 << pass_bits(LOGM, P);
-    static constexpr int L = tw_loaded<R>();
+    static constexpr int Q = tw_quads<R>();
     static constexpr int NB = C::EPT / R;
     static constexpr int NS = 1 << (4 * P);
+    static_assert(NB * Q <= C::EPT / 2, "twiddle registers");
     __device__ __forceinline__ static void run(float4 (&w)[C::EPT / 2], int t, const float2* tw)
     {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const int k = (t + b * C::TPL) & (NS - 1);
-            const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * R);
+            const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * TW_ROW);
 #pragma unroll
-            for (int r2 = 0; r2 < L / 2; ++r2) w[b * (L / 2) + r2] = row[r2];
+            for (int r2 = 0; r2 < Q; ++r2) w[b * Q + r2] = row[r2];
         }
     }
 };
 
-// the R twiddles of one butterfly from its L loaded ones (L/2 float4)
-template <int R, int L>
+// the R twiddles of one butterfly (wt[0] unused) from its row: w^1..w^4
+// loaded, w^(4a+b) = w^(4a) * w^b, w^8 = w^4 * w^4, w^12 = w^8 * w^4 (at most
+// four roundings from the FP64-generated entries)
+template <int R>
 __device__ __forceinline__ void expand_twiddles(float2 (&wt)[R], const float4* q)
 {
-#pragma unroll
-    for (int r2 = 0; r2 < L / 2; ++r2) {
-        wt[2 * r2] = make_float2(q[r2].x, q[r2].y);
-        wt[2 * r2 + 1] = make_float2(q[r2].z, q[r2].w);
+    wt[1] = make_float2(q[0].x, q[0].y);
+    if constexpr (R > 2) {
+        wt[2] = make_float2(q[0].z, q[0].w);
+        wt[3] = make_float2(q[1].x, q[1].y);
     }
-    if constexpr (L < R) {
-        wt[L] = cmul(wt[L / 2], wt[L / 2]);
+    if constexpr (R > 4) {
+        wt[4] = make_float2(q[1].z, q[1].w);
 #pragma unroll
-        for (int a = 2; a * L < R; ++a) wt[a * L] = cmul(wt[(a - 1) * L], wt[L]);
+        for (int b = 1; b < 4; ++b) wt[4 + b] = cmul(wt[4], wt[b]);
+    }
+    if constexpr (R > 8) {
+        wt[8] = cmul(wt[4], wt[4]);
+        wt[12] = cmul(wt[8], wt[4]);
 #pragma unroll
-        for (int a = 1; a * L < R; ++a)
-#pragma unroll
-            for (int b = 1; b < L; ++b) wt[a * L + b] = cmul(wt[a * L], wt[b]);
+        for (int b = 1; b < 4; ++b) {
+            wt[8 + b] = cmul(wt[8], wt[b]);
+            wt[12 + b] = cmul(wt[12], wt[b]);
+        }
     }
 }
 
@@ -446,13 +452,13 @@ struct FftPass {
 #pragma unroll
             for (int r = 0; r < R; ++r) u[r] = v[b + NB * r];
             if constexpr (P > 0) {
-                constexpr int L = tw_loaded<R>();
-                float4 q[L / 2];
-                const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * R);
+                constexpr int Q = tw_quads<R>();
+                float4 q[2];
+                const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * TW_ROW);
 #pragma unroll
-                for (int r2 = 0; r2 < L / 2; ++r2) q[r2] = tw_prefetch<LOGM>() ? w[b * (L / 2) + r2] : row[r2];
+                for (int r2 = 0; r2 < Q; ++r2) q[r2] = tw_prefetch<LOGM>() ? w[b * Q + r2] : row[r2];
                 float2 wt[R];
-                expand_twiddles<R, L>(wt, q);
+                expand_twiddles<R>(wt, q);
 #pragma unroll
                 for (int r = 1; r < R; ++r) u[r] = cmul(u[r], wt[r]);
             }

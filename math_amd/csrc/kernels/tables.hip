@@ -4,7 +4,7 @@
 //   * dtab[q]    = (1/M) * sum_l h2[l] exp(-2*pi*i*(l*q mod M)/M), the FFT of the
 //                  circulant embedding h2 of conj(chirp)  (function.cpp:75-92,
 //                  dft.cpp:77-111), with the 1/M of the inverse FFT folded in
-//   * twiddles   = per-pass Stockham rows exp(-2*pi*i*(k*r mod L)/L)
+//   * twiddles   = per-pass Stockham rows: w^1..w^4 of w = exp(-2*pi*i*k/L), exponents k*e mod L
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -20,13 +20,14 @@ __device__ __forceinline__ double2 unit_root(int64_t q, int64_t den)
     return make_double2(c, -s);
 }
 
+// row k of a pass: w^1..w^TW_ROW of w = exp(-2*pi*i*k/(ns*radix))
 __global__ void k_gen_twiddle_block(float2* out, int ns, int radix)
 {
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t len = int64_t(ns) * radix;
-    if (idx >= len) return;
-    const int64_t k = idx / radix, r = idx % radix;
-    const double2 w = unit_root((k * r) % len, len);
+    if (idx >= int64_t(ns) * TW_ROW) return;
+    const int64_t k = idx / TW_ROW, e = idx % TW_ROW + 1;
+    const double2 w = unit_root((k * e) % len, len);
     out[idx] = make_float2((float)w.x, (float)w.y);
 }
 
@@ -114,7 +115,7 @@ hipError_t gen_twiddles(float2* table, hipStream_t stream)
         for (int p = 1; p < npass_of(logm); ++p) {
             const int ns = 1 << (4 * p);
             const int radix = 1 << pass_bits(logm, p);
-            const int64_t len = int64_t(ns) * radix;
+            const int64_t len = int64_t(ns) * TW_ROW;
             float2* out = table + tw_off(logm) + tw_pass_off(logm, p);
             const int blocks = (int)((len + 255) / 256);
             hipLaunchKernelGGL(k_gen_twiddle_block, dim3(blocks), dim3(256), 0, stream, out, ns, radix);

@@ -39,6 +39,7 @@ LinePlan::~LinePlan()
 {
     if (chirp) (void)hipFree(chirp);
     if (dtab) (void)hipFree(dtab);
+    if (dir) (void)hipFree(dir);
     if (tw_lo) (void)hipFree(tw_lo);
     if (tw_hi) (void)hipFree(tw_hi);
 }
@@ -133,11 +134,16 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
         hip_check(hipMalloc(&lp->dtab, sizeof(float2) * (size_t)m), "hipMalloc dtab");
         lp->bytes += sizeof(float2) * ((size_t)n + (size_t)m);
         double2* chirp64 = reinterpret_cast<double2*>(scratch64_);
-        hip_check(gen_chirp(chirp64, lp->chirp, n, gen_stream_), "chirp generation");
+        if (!lp->large()) {
+            hip_check(hipMalloc(&lp->dir, sizeof(float2) * (size_t)dir_tables_len(n, m)), "hipMalloc tables");
+            lp->bytes += sizeof(float2) * (size_t)dir_tables_len(n, m);
+        }
+        hip_check(gen_chirp(chirp64, lp->chirp, lp->dir, n, m, gen_stream_), "chirp generation");
         if (!lp->large()) {
             // D = FFT_M(h2)/M by a direct FP64 sum with exact index reduction
             hip_check(gen_dtab(lp->dtab, chirp64, scratch64_ + 2 * (size_t)n, n, m, gen_stream_),
                       "diagonal generation");
+            hip_check(gen_dir_diagonal(lp->dir, lp->dtab, n, m, gen_stream_), "diagonal tables");
         } else {
             // long lines: D = FFT_M(h2)/M by this library's four-step FFT on the
             // FP32-rounded embedding (the reference computes its D with an FP32 FFT
@@ -184,8 +190,11 @@ AxisJob Context::make_job(const LinePlan& lp, const float2* src, float2* dst, in
     j.count = count;
     j.inverse = inverse ? 1 : 0;
     j.scale = inverse ? 1.0f / (float)lp.n : 1.0f;
-    j.chirp = lp.chirp;
-    j.dtab = lp.dtab;
+    // Bluestein tables of this direction (tile-resident lengths)
+    const int64_t n = lp.n, m = lp.m;
+    j.chirp = lp.dir ? lp.dir + (inverse ? n : 0) : nullptr;
+    j.dtab = lp.dir ? lp.dir + 2 * n + (inverse ? m : 0) : nullptr;
+    j.chirp_out = lp.dir ? lp.dir + 2 * n + 2 * m + (inverse ? n : 0) : nullptr;
     j.tw = twiddles(lp.logm);
     return j;
 }

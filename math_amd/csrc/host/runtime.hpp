@@ -42,6 +42,8 @@ struct LinePlan {
     bool blue = false;  // Bluestein (n not a power of two)
     float2* chirp = nullptr;  // n entries, Bluestein only
     float2* dtab = nullptr;   // m entries, Bluestein only
+    // tile-resident Bluestein lengths: the direction tables (dir_tables_len)
+    float2* dir = nullptr;
     // lines longer than one tile: M = 2^logm1 * 2^logm2 (four-step), with a
     // two-level table of exp(-2*pi*i*j/M)
     int logm1 = 0, logm2 = 0;

@@ -278,8 +278,10 @@ struct AxisJob {
     int count;             // number of lines
     int inverse;           // 0 forward, 1 inverse
     float scale;           // 1/n for the inverse, 1 otherwise
-    const float2* chirp;   // Bluestein: exp(-i*pi*k^2/n), k < n
-    const float2* dtab;    // Bluestein: FFT_M(conj chirp embedded) / M
+    // Bluestein tables of the job's direction (c = exp(-i*pi*k^2/n)):
+    const float2* chirp;      // load: c (forward) or conj(c) (inverse), k < n
+    const float2* dtab;       // middle: D or conj(D), D = FFT_M(conj(c) embedded) / M
+    const float2* chirp_out;  // store: conj(c) (forward) or c/n (inverse), k < n
     const float2* tw;      // twiddle block of this LOGM
 };
 
@@ -544,8 +546,8 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
                     x = make_float2(image_luminance(io, int64_t(line) * io.ils + int64_t(pos) * io.iel), 0.0f);
                 else
                     x = s[i * step];
-                // forward x*c, inverse x*conj(c) (c = chirp); power of two: conj for the inverse
-                x = BLUE ? cmul(x, cxor(job.chirp[pos], sgn)) : cxor(x, sgn);
+                // forward x*c, inverse x*conj(c) (direction tables); power of two: conj for the inverse
+                x = BLUE ? cmul(x, job.chirp[pos]) : cxor(x, sgn);
             }
             v[i] = x;
         }
@@ -559,7 +561,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
             const uint32_t pos = uint32_t(t + i * C::TPL);
-            v[i] = cmul_conj(v[i], cxor(job.dtab[pos], sgn));
+            v[i] = cmul_conj(v[i], job.dtab[pos]);
         }
         FftTile<LOGM, THREADS, COLS>::run(v, t, ll, lds, job.tw);
     }
@@ -573,8 +575,8 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
             if (in_line((int)pos)) {
                 float2 y;
                 if constexpr (BLUE) {
-                    // forward conj(v)*c = conj(v*conj(c)), inverse conj(v*c)/n
-                    y = cscale(cmul_conj(v[i], cxor(job.chirp[pos], sgn ^ 0x80000000u)), sc);
+                    // forward conj(v)*c = conj(v*conj(c)), inverse conj(v*c)/n = conj(v*(c/n))
+                    y = cmul_conj(v[i], job.chirp_out[pos]);
                 } else {
                     y = cscale(cxor(v[i], sgn), sc);
                 }

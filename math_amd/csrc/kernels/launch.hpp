@@ -58,7 +58,11 @@ hipError_t gen_large_twiddles(float2* lo, float2* hi, int64_t m, hipStream_t str
 hipError_t gen_h2_f32(float2* h2, const double2* chirp64, int64_t n, int64_t m, hipStream_t stream);
 
 hipError_t gen_twiddles(float2* table, hipStream_t stream);
-hipError_t gen_chirp(double2* chirp64, float2* chirp32, int n, hipStream_t stream);
+// Direction tables of a tile-resident Bluestein length (float2 entries):
+// [load fwd c | load inv conj(c) | D | conj(D) | store fwd conj(c) | store inv c/n]
+__host__ __device__ constexpr int64_t dir_tables_len(int64_t n, int64_t m) { return 4 * n + 2 * m; }
+hipError_t gen_chirp(double2* chirp64, float2* chirp32, float2* dir, int n, int m, hipStream_t stream);
+hipError_t gen_dir_diagonal(float2* dir, const float2* dtab, int n, int m, hipStream_t stream);
 hipError_t gen_dtab(float2* dtab, const double2* chirp64, double* cos_scratch, int n, int m, hipStream_t stream);
 
 hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, bool wide, hipStream_t stream);

@@ -31,7 +31,10 @@ __global__ void k_gen_twiddle_block(float2* out, int ns, int radix)
     out[idx] = make_float2((float)w.x, (float)w.y);
 }
 
-__global__ void k_gen_chirp(double2* chirp64, float2* chirp32, int n)
+// chirp c[k] in FP64 and FP32; with dir != nullptr also the direction tables
+// of the tile kernels (layout of dir_tables_len): load c / conj(c), store
+// conj(c) / c/n (the inverse's 1/n folded in, rounded once from FP64)
+__global__ void k_gen_chirp(double2* chirp64, float2* chirp32, float2* dir, int n, int m)
 {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -41,7 +44,25 @@ __global__ void k_gen_chirp(double2* chirp64, float2* chirp32, int n)
     const int64_t q = (int64_t)(((unsigned __int128)kr * kr) % (unsigned __int128)two_n);
     const double2 c = unit_root(q, two_n);
     chirp64[k] = c;
-    chirp32[k] = make_float2((float)c.x, (float)c.y);
+    const float2 c32 = make_float2((float)c.x, (float)c.y);
+    chirp32[k] = c32;
+    if (dir) {
+        const int64_t out = 2 * int64_t(n) + 2 * int64_t(m);
+        dir[k] = c32;
+        dir[n + k] = make_float2(c32.x, -c32.y);
+        dir[out + k] = make_float2(c32.x, -c32.y);
+        dir[out + n + k] = make_float2((float)(c.x / (double)n), (float)(c.y / (double)n));
+    }
+}
+
+// the diagonal and its conjugate into the direction tables
+__global__ void k_dir_diagonal(float2* dir, const float2* dtab, int n, int m)
+{
+    const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const float2 d = dtab[q];
+    dir[2 * int64_t(n) + q] = d;
+    dir[2 * int64_t(n) + m + q] = make_float2(d.x, -d.y);
 }
 
 // cos(2*pi*q/m), q < m
@@ -124,10 +145,17 @@ hipError_t gen_twiddles(float2* table, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t gen_chirp(double2* chirp64, float2* chirp32, int n, hipStream_t stream)
+hipError_t gen_chirp(double2* chirp64, float2* chirp32, float2* dir, int n, int m, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_gen_chirp, dim3((unsigned)((int64_t(n) + 255) / 256)), dim3(256), 0, stream, chirp64, chirp32,
-                       n);
+                       dir, n, m);
+    return hipGetLastError();
+}
+
+hipError_t gen_dir_diagonal(float2* dir, const float2* dtab, int n, int m, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_dir_diagonal, dim3((unsigned)((int64_t(m) + 255) / 256)), dim3(256), 0, stream, dir, dtab, n,
+                       m);
     return hipGetLastError();
 }
 

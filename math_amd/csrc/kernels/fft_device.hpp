@@ -97,9 +97,6 @@ __host__ __device__ constexpr int64_t tw_off(int logm)
 }
 __host__ __device__ constexpr int64_t tw_total() { return tw_off(MAX_LOGM + 1); }
 
-#ifndef MDFT_LDS_LAYOUT
-#define MDFT_LDS_LAYOUT 1
-#endif
 // LDS word of line position p in the exchange written by a pass of stride NS.
 // A pass writes its 16 outputs NS positions apart and the next pass reads 16
 // positions TPL apart.  When a 32-lane half-wave covers 32 consecutive
@@ -112,7 +109,7 @@ __host__ __device__ constexpr int64_t tw_total() { return tw_off(MAX_LOGM + 1); 
 template <int NS, bool ONE_LINE>
 __host__ __device__ constexpr int xpad(int p)
 {
-    if constexpr (!ONE_LINE || !MDFT_LDS_LAYOUT) {
+    if constexpr (!ONE_LINE) {
         return p + (p >> 4);
     } else if constexpr (NS == 1) {
         return p + (p >> 5);
@@ -328,9 +325,6 @@ __device__ __forceinline__ float cabs_scaled(float2 y, float s)
 // ---------------------------------------------------------------------------
 // FFT of the tile held in registers
 // ---------------------------------------------------------------------------
-#ifndef MDFT_WAVE_LOCAL_SYNC
-#define MDFT_WAVE_LOCAL_SYNC 1
-#endif
 // Barrier around an LDS exchange.  When every line of a tile lies inside one
 // wave (row mapping with TPL <= 64), an exchange never crosses waves, and the
 // LDS operations of one wave are performed in order, so a wave-level
@@ -348,21 +342,14 @@ __device__ __forceinline__ void exchange_sync()
     }
 }
 
-#ifndef MDFT_TW_PREFETCH
-#define MDFT_TW_PREFETCH 1
-#endif
-// Twiddle rows of pass P for the thread's butterflies: NB rows of R/2 float4
-// (EPT/2 in all).  Loaded one pass ahead (pass 1's with the tile's input), so
-// the table latency overlaps the LDS exchange instead of following it.
-// Off for padded lengths above 4096 (four passes), where the extra registers
-// spill at 1024 threads.
-#ifndef MDFT_TW_PREFETCH_MAXLOGM
-#define MDFT_TW_PREFETCH_MAXLOGM 12
-#endif
+// Twiddle rows of pass P for the thread's butterflies are loaded one pass
+// ahead (pass 1's with the tile's input), so the table latency overlaps the
+// LDS exchange instead of following it.  Off for padded lengths above 4096
+// (four passes), where the extra registers spill at 1024 threads.
 template <int LOGM>
 __host__ __device__ constexpr bool tw_prefetch()
 {
-    return MDFT_TW_PREFETCH && LOGM <= MDFT_TW_PREFETCH_MAXLOGM;
+    return LOGM <= 12;
 }
 
 // float4s of a row a radix-R butterfly reads: (w^1, w^2) and (w^3, w^4)
@@ -422,7 +409,7 @@ __device__ __forceinline__ void expand_twiddles(float2 (&wt)[R], const float4* q
 template <int LOGM, int THREADS, bool COLS, int P>
 struct FftPass {
     using C = LineCfg<LOGM, THREADS>;
-    static constexpr bool WAVE = MDFT_WAVE_LOCAL_SYNC && !COLS && C::TPL <= 64;
+    static constexpr bool WAVE = !COLS && C::TPL <= 64;
     static constexpr int RB = pass_bits(LOGM, P);
     static constexpr int R = 1 << RB;
     static constexpr int NB = C::EPT / R;

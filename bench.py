@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--requests", type=int, default=20000, help="serving stream length (0: skip serving)")
     ap.add_argument("--serve-budget-gib", type=float, default=8.0)
     ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--single-only", action="store_true",
+                    help="only time the single-image configurations (for profiling them)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (scale for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -108,6 +110,10 @@ def main():
 
     M.init(0)
     dev = torch.device("cuda:0")
+    if args.single_only:
+        M.tune_auto()
+        print(json.dumps({"single": single_transforms(M, torch)}), flush=True)
+        return
     sizes, invs = mixed_batch_sizes()
     offs, total = [], 0
     for (w, h) in sizes:

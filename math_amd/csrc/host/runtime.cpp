@@ -283,7 +283,7 @@ void Context::enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, i
         return;
     }
     const AxisJob j = make_job(lp, src, dst, es, ls, count, inverse);
-    hip_check(launch_axis(lp.logm, j, lp.blue, cols, wide_tiles(lp.logm, lp.blue, cols, count), stream),
+    hip_check(launch_axis(lp.logm, j, lp.blue, cols, tile_threads(lp.logm, lp.blue, cols, count), stream),
               "axis pass launch");
 }
 
@@ -295,6 +295,19 @@ bool Context::wide_tiles(int logm, bool blue, bool cols, int count) const
     const int lines_wide = tpl >= 1024 ? 1 : 1024 / tpl;
     const int64_t tiles = (int64_t(count) + lines_wide - 1) / lines_wide;
     return tiles >= int64_t(cu_count_);
+}
+
+int Context::tile_threads(int logm, bool blue, bool cols, int count) const
+{
+    if (wide_tiles(logm, blue, cols, count)) return 1024;
+    // passes with few lines: smaller tiles (one wave for lines of up to 1024
+    // points) so every CU gets work; rows aim at four tiles per CU, columns at
+    // one, since narrower column tiles read shorter runs of each row
+    const int tpl = (1 << logm) / ept_of(logm);
+    const int64_t want = int64_t(cols ? 1 : 4) * cu_count_;
+    int t = tpl > 256 ? tpl : 256;
+    while (t > 64 && t / 2 >= tpl && (int64_t(count) + t / tpl - 1) / (t / tpl) < want) t /= 2;
+    return t;
 }
 
 void Context::enqueue_transform(const LinePlan& row, const LinePlan& col, int w, int h, bool inverse,

@@ -94,6 +94,8 @@ public:
     int variant(int logm, bool blue, bool cols) const { return variant_[logm][blue][cols]; }
     // the table's choice, narrowed when wide tiles would leave CUs without work
     bool wide_tiles(int logm, bool blue, bool cols, int count) const;
+    // workgroup size of a single-image pass of `count` lines (launch_axis)
+    int tile_threads(int logm, bool blue, bool cols, int count) const;
     void set_variant(int logm, bool blue, bool cols, int v) { variant_[logm][blue][cols] = (uint8_t)v; }
 
 private:

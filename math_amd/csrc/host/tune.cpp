@@ -76,7 +76,7 @@ int mdft_tune_auto(double* seconds)
                     int best_v = 0;
                     for (int v = 0; v < 2; ++v) {
                         const float ms = time_launches(
-                            s, [&]() { mdft::hip_check(mdft::launch_axis(logm, j, blue != 0, cols != 0, v != 0, s),
+                            s, [&]() { mdft::hip_check(mdft::launch_axis(logm, j, blue != 0, cols != 0, v ? 1024 : 256, s),
                                                        "tuning launch"); },
                             5);
                         if (v == 0 || ms < best) {

@@ -54,31 +54,40 @@ static hipError_t launch_axis_tt(const AxisJob& job, bool blue, bool cols, hipSt
     return hipGetLastError();
 }
 
+// threads: 1024 = wide tiles, 256 = narrow, 128 / 64 = small tiles for
+// passes with few lines (clamped up to one line per workgroup)
 template <int LOGM>
-static hipError_t launch_axis_t(const AxisJob& job, bool blue, bool cols, bool wide, hipStream_t stream)
+static hipError_t launch_axis_t(const AxisJob& job, bool blue, bool cols, int threads, hipStream_t stream)
 {
+    constexpr int TPL = (1 << LOGM) / ept_of(LOGM);
     constexpr int TN = axis_threads(LOGM, false), TW = axis_threads(LOGM, true);
-    if (wide && TW != TN) return launch_axis_tt<LOGM, TW>(job, blue, cols, stream);
+    if (threads >= 1024 && TW != TN) return launch_axis_tt<LOGM, TW>(job, blue, cols, stream);
+    if constexpr (TPL <= 64) {
+        if (threads <= 64) return launch_axis_tt<LOGM, 64>(job, blue, cols, stream);
+    }
+    if constexpr (TPL <= 128) {
+        if (threads <= 128) return launch_axis_tt<LOGM, 128>(job, blue, cols, stream);
+    }
     return launch_axis_tt<LOGM, TN>(job, blue, cols, stream);
 }
 
-hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, bool wide, hipStream_t stream)
+hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, int threads, hipStream_t stream)
 {
     switch (logm) {
-    case 1: return launch_axis_t<1>(job, blue, cols, wide, stream);
-    case 2: return launch_axis_t<2>(job, blue, cols, wide, stream);
-    case 3: return launch_axis_t<3>(job, blue, cols, wide, stream);
-    case 4: return launch_axis_t<4>(job, blue, cols, wide, stream);
-    case 5: return launch_axis_t<5>(job, blue, cols, wide, stream);
-    case 6: return launch_axis_t<6>(job, blue, cols, wide, stream);
-    case 7: return launch_axis_t<7>(job, blue, cols, wide, stream);
-    case 8: return launch_axis_t<8>(job, blue, cols, wide, stream);
-    case 9: return launch_axis_t<9>(job, blue, cols, wide, stream);
-    case 10: return launch_axis_t<10>(job, blue, cols, wide, stream);
-    case 11: return launch_axis_t<11>(job, blue, cols, wide, stream);
-    case 12: return launch_axis_t<12>(job, blue, cols, wide, stream);
-    case 13: return launch_axis_t<13>(job, blue, cols, wide, stream);
-    case 14: return launch_axis_t<14>(job, blue, cols, wide, stream);
+    case 1: return launch_axis_t<1>(job, blue, cols, threads, stream);
+    case 2: return launch_axis_t<2>(job, blue, cols, threads, stream);
+    case 3: return launch_axis_t<3>(job, blue, cols, threads, stream);
+    case 4: return launch_axis_t<4>(job, blue, cols, threads, stream);
+    case 5: return launch_axis_t<5>(job, blue, cols, threads, stream);
+    case 6: return launch_axis_t<6>(job, blue, cols, threads, stream);
+    case 7: return launch_axis_t<7>(job, blue, cols, threads, stream);
+    case 8: return launch_axis_t<8>(job, blue, cols, threads, stream);
+    case 9: return launch_axis_t<9>(job, blue, cols, threads, stream);
+    case 10: return launch_axis_t<10>(job, blue, cols, threads, stream);
+    case 11: return launch_axis_t<11>(job, blue, cols, threads, stream);
+    case 12: return launch_axis_t<12>(job, blue, cols, threads, stream);
+    case 13: return launch_axis_t<13>(job, blue, cols, threads, stream);
+    case 14: return launch_axis_t<14>(job, blue, cols, threads, stream);
     default: return hipErrorInvalidValue;
     }
 }

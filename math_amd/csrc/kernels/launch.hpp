@@ -65,7 +65,8 @@ hipError_t gen_chirp(double2* chirp64, float2* chirp32, float2* dir, int n, int 
 hipError_t gen_dir_diagonal(float2* dir, const float2* dtab, int n, int m, hipStream_t stream);
 hipError_t gen_dtab(float2* dtab, const double2* chirp64, double* cos_scratch, int n, int m, hipStream_t stream);
 
-hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, bool wide, hipStream_t stream);
+// threads: 1024 wide tiles, 256 narrow, 128 or 64 small (clamped to one line per workgroup)
+hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, int threads, hipStream_t stream);
 
 // image path: an axis pass with fused luminance input and/or magnitude output
 // (mode = IO_LUM | IO_MAG bits), and the unfused conversions

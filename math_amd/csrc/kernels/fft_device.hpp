@@ -344,12 +344,12 @@ __device__ __forceinline__ void exchange_sync()
 
 // Twiddle rows of pass P for the thread's butterflies are loaded one pass
 // ahead (pass 1's with the tile's input), so the table latency overlaps the
-// LDS exchange instead of following it.  Off for padded lengths above 4096
-// (four passes), where the extra registers spill at 1024 threads.
+// LDS exchange instead of following it.  Off for 16384 points, where the
+// extra registers spill.
 template <int LOGM>
 __host__ __device__ constexpr bool tw_prefetch()
 {
-    return LOGM <= 12;
+    return LOGM <= 13;
 }
 
 // float4s of a row a radix-R butterfly reads: (w^1, w^2) and (w^3, w^4)

@@ -61,7 +61,9 @@ typedef struct mdft_plan_s* mdft_plan;
  * ComputeVector::create_buffers(width, height) (src/gpu/dft/compute.h:56,
  * compute.cpp:229-255) and Dft::create_buffers (dft.h:38, dft.cpp:262-309).
  * Errors: MDFT_ERR_SIZE with "Error size <n> for compute m" when a side is
- * outside [1, 2^29]. */
+ * outside [1, 2^29], and "... exceeds the longest supported line (2^28)" when
+ * its padded length is above 2^28 (a side of 2^29, or a non-power-of-two side
+ * above 2^27 + 1). */
 int mdft_plan_create(mdft_plan* plan, uint32_t width, uint32_t height);
 
 /* Free a plan and its device memory (ComputeVector destructor /

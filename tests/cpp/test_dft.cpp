@@ -8,14 +8,17 @@
 // the normalised-L2 bound 2e-6*log2(padded N).
 #include <chrono>
 #include <cmath>
+#include <filesystem>
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <random>
 #include <string>
 #include <vector>
 
 #include "../../include/mdft.hpp"
+#include "../../include/mdft_data.hpp"
 #include "../../oracle/oracle.h"
 
 using Complex = std::complex<float>;
@@ -122,6 +125,65 @@ void constant_data_test(mdft::ComputeVector* dft)
         if (std::abs(data[i] - expect[i]) > 1e-5f) fail("constant: closed form");
 }
 
+// test_dft.cpp:409-430 and :222: the random input goes through the reference's
+// text file format (generate, load), the forward output is saved the same way
+void data_file_test(mdft::ComputeVector* dft, int n1, int n2)
+{
+    std::printf("----- Random Data File DFT Test -----\n");
+    const auto dir = std::filesystem::temp_directory_path();
+    const auto in = dir / ("mdft_dft_input_" + std::to_string(n1) + "x" + std::to_string(n2) + ".txt");
+    const auto out = dir / ("mdft_dft_output_" + std::to_string(n1) + "x" + std::to_string(n2) + ".txt");
+    mdft::data::generate_random_data<float>(in, n1, n2);
+    const mdft::data::LoadData<float> d = mdft::data::load_data<float>(in);
+    if (d.n1 != n1 || d.n2 != n2) fail("data file: dimensions");
+    run("file", dft, d.n1, d.n2, d.data);
+    std::vector<Complex> forward(d.data);
+    dft->create_buffers(n1, n2);
+    dft->exec(false, &forward);
+    mdft::data::save_data(out, forward);
+    std::vector<Complex> back(forward.size());
+    {
+        // the saved file re-read with a header holds exactly the computed values
+        std::ifstream f(out);
+        for (auto& c : back) {
+            float re, im;
+            f >> re >> im;
+            c = {re, im};
+        }
+    }
+    if (back != forward) fail("data file: saved output does not read back exactly");
+    std::filesystem::remove(in);
+    std::filesystem::remove(out);
+}
+
+// the batched power-of-two FFT (src/gpu/dft/fft.h:32-53) against the FP64 oracle, per line
+void fft1d_test()
+{
+    std::printf("----- Batched 1-D FFT -----\n");
+    const int count = 7, n = 512;
+    std::vector<Complex> data = uniform_data(n, count, 4242);
+    const std::vector<Complex> source(data);
+    mdft::Fft fft(count, n);
+    for (int inverse = 0; inverse < 2; ++inverse) {
+        data = source;
+        fft.run_for_data(inverse != 0, &data);
+        double worst = 0;
+        for (int j = 0; j < count; ++j) {
+            std::vector<double> in(2 * n), ref(2 * n);
+            for (int k = 0; k < n; ++k) {
+                in[2 * k] = source[(size_t)j * n + k].real();
+                in[2 * k + 1] = source[(size_t)j * n + k].imag();
+            }
+            orc_dft1d_64(n, inverse, in.data(), ref.data());
+            double m[4];
+            orc_error_metrics(n, reinterpret_cast<const float*>(data.data() + (size_t)j * n), ref.data(), m);
+            worst = std::max(worst, m[0]);
+        }
+        std::printf("  %s: worst normalised L2 vs FP64 %.3e\n", inverse ? "inverse" : "forward", worst);
+        if (worst > 2e-6 * std::log2((double)n)) fail("batched FFT: normalised L2 above the bound");
+    }
+}
+
 void error_tests()
 {
     std::printf("----- Error behaviour -----\n");
@@ -161,6 +223,8 @@ int main(int argc, char** argv)
         run("random", dft.get(), w, h, uniform_data(w, h, (unsigned)(w * 1000 + h)));
     }
     if (large) run("random", dft.get(), 6007, 997, uniform_data(6007, 997, 997));
+    data_file_test(dft.get(), 37, 23);
+    fft1d_test();
     error_tests();
     std::printf(failures ? "DFT check FAILED (%d)\n" : "DFT check passed\n", failures);
     return failures ? 1 : 0;

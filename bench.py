@@ -251,7 +251,8 @@ def single_transforms(M, torch, reps=20):
     """One image per call on device memory (mdft_exec_device), event-timed: the
     single-image configurations 640x480 (forward and inverse), 1024x1024 (round
     trip) and 4093x2047, with their algorithmic bandwidth 32*w*h bytes per
-    transform."""
+    transform; also the image path on a 640x480 rectangle and 4096 FFTs of
+    4096 points (16 bytes per point)."""
     res = {}
     sh = torch.cuda.current_stream().cuda_stream
     g = torch.Generator(device="cuda:0")
@@ -275,7 +276,32 @@ def single_transforms(M, torch, reps=20):
                                                           "alg_gbs": round(32.0 * w * h / (us * 1e-6) / 1e9, 1)}
         p.close()
         del x, y
+    # the view's image path (RGBA32F rectangle -> |X|/(w*h) R32F) and the batched 1-D FFT
+    img = M.Image(640, 480, M.PIXEL_RGBA32F)
+    pix = torch.rand((500, 700, 4), dtype=torch.float32, device="cuda:0", generator=g)
+    mag = torch.empty((480, 640), dtype=torch.float32, device="cuda:0")
+    run_img = lambda: img.exec_device(pix.data_ptr(), 700, 500, 700, 30, 10, mag.data_ptr(), 640, sh)
+    res["image_640x480_rgba32f"] = {"us": round(_event_us(torch, run_img, reps), 1)}
+    img.close()
+    f = M.Fft(4096, 4096)
+    z = torch.randn(4096 * 4096, dtype=torch.complex64, device="cuda:0", generator=g)
+    us = _event_us(torch, lambda: f.exec_device(False, z.data_ptr(), z.data_ptr(), sh), reps)
+    res["fft1d_4096x4096"] = {"us": round(us, 1), "alg_gbs": round(16.0 * z.numel() / (us * 1e-6) / 1e9, 1)}
+    f.close()
     return res
+
+
+def _event_us(torch, fn, reps):
+    for _ in range(3):
+        fn()
+    s0 = torch.cuda.Event(enable_timing=True)
+    s1 = torch.cuda.Event(enable_timing=True)
+    s0.record()
+    for _ in range(reps):
+        fn()
+    s1.record()
+    torch.cuda.synchronize()
+    return s0.elapsed_time(s1) * 1e3 / reps
 
 
 def plain_paths(M, torch, src, dst, sizes, invs, offs, flops):

@@ -79,7 +79,8 @@ float time_column_phase(Context& ctx, const std::vector<mdft::BatchEntry>& entri
     float ms = 0.0f;
     hipError_t err = hipMalloc(&jobs, sizeof(BatchJob) * bb.jobs.size());
     if (err == hipSuccess) err = hipMalloc(&items, sizeof(uint2) * n2);
-    if (err == hipSuccess) err = hipMemcpy(jobs, bb.jobs.data(), sizeof(BatchJob) * bb.jobs.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(jobs, bb.jobs.data(), sizeof(BatchJob) * bb.jobs.size(), hipMemcpyHostToDevice);
     if (err == hipSuccess)
         err = hipMemcpy(items, bb.items.data() + bb.nphase1, sizeof(uint2) * n2, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipEventCreate(&a);

@@ -104,7 +104,8 @@ int mdft_exec(mdft_plan plan, int inverse, void* data, size_t count)
     if (!data || count != (size_t)plan->w * plan->h) throw Error(MDFT_ERR_DATA_SIZE, "Wrong DFT buffer size");
     mdft::DeviceGuard g(plan->device);
     const size_t bytes = sizeof(float2) * count;
-    if (!plan->stream) mdft::hip_check(hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (!plan->stream)
+        mdft::hip_check(hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (!plan->xd) mdft::hip_check(hipMalloc(&plan->xd, bytes), "hipMalloc image");
     Context& ctx = Context::get(plan->device);
     mdft::hip_check(hipMemcpyAsync(plan->xd, data, bytes, hipMemcpyHostToDevice, plan->stream), "copy in");

@@ -119,9 +119,8 @@ int mdft_image_exec_device(mdft_image image, const void* src, uint32_t src_width
         io.mel = (int64_t)dst_pitch, io.mls = 1;
         const mdft::AxisJob j = ctx.make_job(col, image->buf, nullptr, w, 1, w, false);
         const int mode = mdft::IO_MAG | (w == 1 ? mdft::IO_LUM : 0);
-        launch(mdft::launch_image_axis(col.logm, j, io, col.blue, true, mode, ctx.wide_tiles(col.logm, col.blue, true, w),
-                                       s),
-               "image column pass launch");
+        const bool wide = ctx.wide_tiles(col.logm, col.blue, true, w);
+        launch(mdft::launch_image_axis(col.logm, j, io, col.blue, true, mode, wide, s), "image column pass launch");
     }
     MDFT_API_END
 }

@@ -75,10 +75,12 @@ int mdft_tune_auto(double* seconds)
                     float best = 0;
                     int best_v = 0;
                     for (int v = 0; v < 2; ++v) {
-                        const float ms = time_launches(
-                            s, [&]() { mdft::hip_check(mdft::launch_axis(logm, j, blue != 0, cols != 0, v ? 1024 : 256, s),
-                                                       "tuning launch"); },
-                            5);
+                        const int threads = v ? 1024 : 256;
+                        auto launch = [&]() {
+                            mdft::hip_check(mdft::launch_axis(logm, j, blue != 0, cols != 0, threads, s),
+                                            "tuning launch");
+                        };
+                        const float ms = time_launches(s, launch, 5);
                         if (v == 0 || ms < best) {
                             best = ms;
                             best_v = v;
@@ -144,7 +146,9 @@ int mdft_tune_load(const char* path)
         entries.push_back(e);
     }
     std::fclose(f);
-    if (bad) throw Error(MDFT_ERR_INVALID, std::string("tune: bad entry on line ") + std::to_string(lineno) + " of " + path);
+    if (bad)
+        throw Error(MDFT_ERR_INVALID,
+                    std::string("tune: bad entry on line ") + std::to_string(lineno) + " of " + path);
     // validated as a whole before anything is applied
     Context& ctx = Context::get(Context::current_device());
     for (const Entry& e : entries) ctx.set_variant(e.logm, e.blue != 0, e.cols != 0, e.v);

@@ -203,7 +203,10 @@ int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count,
 /* Enqueue the whole batch on a stream.  Default shape: two launches (small
  * images whole plus row tiles, then column tiles), one workgroup per item;
  * the flags select persistent or dependency-counter shapes.  src may equal
- * dst (in place per transform). */
+ * dst (in place per transform).  A call whose src/dst differ from the
+ * previous call's re-uploads the job table after synchronising the device;
+ * make that call outside stream capture, after which calls with the same
+ * pointers may be captured into a caller's graph. */
 int mdft_batch_exec_device(mdft_batch batch, const void* src, void* dst, void* stream);
 
 /* Device error flag of the last completed launch: 0, or MDFT_ERR_TIMEOUT if

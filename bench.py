@@ -12,7 +12,7 @@ of the image).
 
 Prints one JSON line.  Also reported: p50/p99 step latency, the plain
 one-transform-per-call paths (host vectors, and device-resident), accuracy of
-sampled transforms against the FP64 oracle, and the CPU baseline (the FP32
+sampled transforms against numpy's FP64 FFT, and the CPU baseline (the FP32
 restatement of the reference pipeline, threaded over lines, on a bounded
 sample of the same workload).
 """
@@ -225,10 +225,33 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def accuracy_sample(M, torch, src, dst, sizes, invs, offs, count=12):
-    """Normalised L2 / max error of sampled batch outputs against the FP64 oracle."""
-    from oracle import oracle as O
+def _padded(n):
+    """Padded FFT length of one side (src/gpu/dft/function.cpp:32-45)."""
+    if n & (n - 1) == 0:
+        return n
+    m = 1
+    while m < 2 * n - 2:
+        m <<= 1
+    return m
 
+
+def _errors(got, ref):
+    """Normalised L2, normalised max and relative L1 of got against an FP64 reference."""
+    d = got.astype(np.complex128) - ref
+    nref = float(np.linalg.norm(ref))
+    return {"nl2": float(np.linalg.norm(d)) / nref, "nmax": float(np.abs(d).max() / np.abs(ref).max()),
+            "rel_l1": float(np.abs(d).sum() / np.abs(got.astype(np.complex128)).sum())}
+
+
+def _fft2_64(x, inverse):
+    """numpy's FP64 2-D DFT with the reference's scaling (forward unnormalised, inverse 1/(w*h))."""
+    x = x.astype(np.complex128)
+    return np.fft.ifft2(x) if inverse else np.fft.fft2(x)
+
+
+def accuracy_sample(M, torch, src, dst, sizes, invs, offs, count=12):
+    """Normalised L2 / max error of sampled batch outputs against numpy's FP64 FFT
+    (bound 2e-6 * log2(padded N); tol_ratio = worst error / bound)."""
     srcn = src.cpu().numpy()
     dstn = dst.cpu().numpy()
     idx = sorted(range(len(sizes)), key=lambda i: sizes[i][0] * sizes[i][1])
@@ -238,13 +261,12 @@ def accuracy_sample(M, torch, src, dst, sizes, invs, offs, count=12):
         w, h = sizes[i]
         x = srcn[offs[i]:offs[i] + w * h].reshape(h, w)
         y = dstn[offs[i]:offs[i] + w * h].reshape(h, w)
-        m = O.error_metrics(y, O.dft2d_64(x, invs[i], nthreads=8))
-        tol = 2e-6 * max(1.0, math.log2(O.compute_m(w) * O.compute_m(h)))
-        worst["nl2"] = max(worst["nl2"], m["nl2"])
-        worst["nmax"] = max(worst["nmax"], m["nmax"])
-        worst["rel_l1"] = max(worst["rel_l1"], m["rel_l1"])
+        m = _errors(y, _fft2_64(x, invs[i]))
+        tol = 2e-6 * max(1.0, math.log2(_padded(w) * _padded(h)))
+        for k in ("nl2", "nmax", "rel_l1"):
+            worst[k] = max(worst[k], m[k])
         worst["tol_ratio"] = max(worst["tol_ratio"], m["nl2"] / tol)
-    return {k: float("%.3e" % v) for k, v in worst.items()} | {"sampled": len(pick), "oracle": "fp64"}
+    return {k: float("%.3e" % v) for k, v in worst.items()} | {"sampled": len(pick), "reference": "numpy fp64 fft"}
 
 
 def single_transforms(M, torch, reps=20):
@@ -272,8 +294,21 @@ def single_transforms(M, torch, reps=20):
             s1.record()
             torch.cuda.synchronize()
             us = s0.elapsed_time(s1) * 1e3 / reps
-            res[f"{w}x{h}_{'inv' if inv else 'fwd'}"] = {"us": round(us, 1),
-                                                          "alg_gbs": round(32.0 * w * h / (us * 1e-6) / 1e9, 1)}
+            xin = x.cpu().numpy().reshape(h, w)
+            m = _errors(y.cpu().numpy().reshape(h, w), _fft2_64(xin, inv))
+            res[f"{w}x{h}_{'inv' if inv else 'fwd'}"] = {
+                "us": round(us, 1), "alg_gbs": round(32.0 * w * h / (us * 1e-6) / 1e9, 1),
+                "nl2": float("%.3e" % m["nl2"]), "nmax": float("%.3e" % m["nmax"]),
+                "bound": float("%.3e" % (2e-6 * max(1.0, math.log2(_padded(w) * _padded(h)))))}
+        if len(invs) == 2:
+            # round trip forward then inverse
+            z = torch.empty_like(x)
+            p.exec_device(False, x.data_ptr(), y.data_ptr(), sh)
+            p.exec_device(True, y.data_ptr(), z.data_ptr(), sh)
+            torch.cuda.synchronize()
+            rt = _errors(z.cpu().numpy(), x.cpu().numpy().astype(np.complex128))
+            res[f"{w}x{h}_round_trip"] = {"nl2": float("%.3e" % rt["nl2"]), "nmax": float("%.3e" % rt["nmax"])}
+            del z
         p.close()
         del x, y
     # the view's image path (RGBA32F rectangle -> |X|/(w*h) R32F) and the batched 1-D FFT

@@ -223,37 +223,46 @@ __host__ __device__ constexpr int bitrev_small(int v, int bits)
     return r;
 }
 
-// one radix-2 DIF stage of span HALF over R points
-template <int R, int HALF, int BASE, int K>
+// one radix-2 DIF stage of span HALF over R points.  ZHI (first stage
+// only): u[R/2 + 1 ..] are known to be zero, so every pair but the first
+// is (a, 0) and becomes (a, twiddle * a) without additions.
+template <int R, int HALF, int BASE, int K, bool ZHI>
 __device__ __forceinline__ void dif_bfly(float2* u)
 {
     if constexpr (BASE < R) {
         if constexpr (K < HALF) {
-            const float2 a = u[BASE + K], b = u[BASE + K + HALF];
-            u[BASE + K] = cadd(a, b);
-            u[BASE + K + HALF] = twc<K*(16 / (2 * HALF))>(csub(a, b));
-            dif_bfly<R, HALF, BASE, K + 1>(u);
+            const float2 a = u[BASE + K];
+            if constexpr (ZHI && K > 0) {
+                u[BASE + K + HALF] = twc<K*(16 / (2 * HALF))>(a);
+            } else {
+                const float2 b = u[BASE + K + HALF];
+                u[BASE + K] = cadd(a, b);
+                u[BASE + K + HALF] = twc<K*(16 / (2 * HALF))>(csub(a, b));
+            }
+            dif_bfly<R, HALF, BASE, K + 1, ZHI>(u);
         } else {
-            dif_bfly<R, HALF, BASE + 2 * HALF, 0>(u);
+            dif_bfly<R, HALF, BASE + 2 * HALF, 0, ZHI>(u);
         }
     }
 }
 
-template <int R, int HALF>
+template <int R, int HALF, bool ZHI>
 __device__ __forceinline__ void dif_stages(float2* u)
 {
     if constexpr (HALF >= 1) {
-        dif_bfly<R, HALF, 0, 0>(u);
-        dif_stages<R, HALF / 2>(u);
+        dif_bfly<R, HALF, 0, 0, ZHI>(u);
+        dif_stages<R, HALF / 2, false>(u);
     }
 }
 
-template <int R>
+// ZHI: inputs u[R/2 + 1 ..] are zero (the zero-padded half of a Bluestein
+// line in its first pass)
+template <int R, bool ZHI = false>
 __device__ __forceinline__ void dft_fwd(float2* u)
 {
     static_assert(R >= 1 && R <= 16 && (R & (R - 1)) == 0, "radix");
     if constexpr (R > 1) {
-        dif_stages<R, R / 2>(u);
+        dif_stages<R, R / 2, ZHI>(u);
         constexpr int bits = (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : 4;
         float2 t[R];
 #pragma unroll
@@ -406,7 +415,8 @@ __device__ __forceinline__ void expand_twiddles(float2 (&wt)[R], const float4* q
     }
 }
 
-template <int LOGM, int THREADS, bool COLS, int P>
+// ZIN (first pass only): slots EPT/2 + 1 .. EPT of every thread are zero.
+template <int LOGM, int THREADS, bool COLS, int P, bool ZIN = false>
 struct FftPass {
     using C = LineCfg<LOGM, THREADS>;
     static constexpr bool WAVE = !COLS && C::TPL <= 64;
@@ -451,7 +461,8 @@ struct FftPass {
 #pragma unroll
                 for (int r = 1; r < R; ++r) u[r] = cmul(u[r], wt[r]);
             }
-            dft_fwd<R>(u);
+            static_assert(!ZIN || (P == 0 && NB == 1), "zero inputs only in a single-butterfly first pass");
+            dft_fwd<R, ZIN>(u);
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b + NB * r] = u[r];
         }
@@ -482,7 +493,10 @@ struct FftPass {
     }
 };
 
-template <int LOGM, int THREADS, bool COLS>
+// ZIN: the upper half of the line is zero apart from position M/2, so slots
+// EPT/2 + 1 .. EPT of every thread hold zeros (the caller set them to
+// constant zero).
+template <int LOGM, int THREADS, bool COLS, bool ZIN = false>
 struct FftTile {
     using C = LineCfg<LOGM, THREADS>;
     __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, int ll, float* lds, const float2* tw)
@@ -492,7 +506,7 @@ struct FftTile {
         float* im = lds + C::LPW * LST + ll * LST;
         float4 w[C::EPT / 2];
         if constexpr (tw_prefetch<LOGM>() && C::NPASS > 1) TwLoad<LOGM, THREADS, 1>::run(w, t, tw);
-        FftPass<LOGM, THREADS, COLS, 0>::run(v, t, re, im, tw, w);
+        FftPass<LOGM, THREADS, COLS, 0, ZIN>::run(v, t, re, im, tw, w);
     }
 };
 
@@ -520,6 +534,10 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
     const uint32_t sgn = job.inverse ? 0x80000000u : 0u;
     // a power-of-two line fills the FFT (n == M); only Bluestein pads
     auto in_line = [&](int pos) { return !BLUE || pos < n; };
+    // Bluestein: n <= M/2 + 1, so slot i holds a position of the line only
+    // for i <= EPT/2 (slot EPT/2 only at t = 0 when n = M/2 + 1); the other
+    // slots are zero on input and not stored on output.
+    constexpr int LIVE = BLUE ? C::EPT / 2 + 1 : C::EPT;
 
     float2 v[C::EPT];
     {
@@ -528,7 +546,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
         for (int i = 0; i < C::EPT; ++i) {
             const uint32_t pos = uint32_t(t + i * C::TPL);
             float2 x = make_float2(0.0f, 0.0f);
-            if (valid && in_line((int)pos)) {
+            if (i < LIVE && valid && in_line((int)pos)) {
                 if constexpr ((IO & IO_LUM) != 0)
                     x = make_float2(image_luminance(io, int64_t(line) * io.ils + int64_t(pos) * io.iel), 0.0f);
                 else
@@ -540,7 +558,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
         }
     }
 
-    FftTile<LOGM, THREADS, COLS>::run(v, t, ll, lds, job.tw);
+    FftTile<LOGM, THREADS, COLS, BLUE>::run(v, t, ll, lds, job.tw);
 
     if constexpr (BLUE) {
         // forward conj(v*D), inverse conj(v)*D = conj(v*conj(D)); the outer
@@ -557,7 +575,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
         float2* o = job.dst + int64_t(line) * job.ls + int64_t(t) * es;
         const float sc = job.scale;
 #pragma unroll
-        for (int i = 0; i < C::EPT; ++i) {
+        for (int i = 0; i < LIVE; ++i) {
             const uint32_t pos = uint32_t(t + i * C::TPL);
             if (in_line((int)pos)) {
                 float2 y;

@@ -416,7 +416,10 @@ __device__ __forceinline__ void expand_twiddles(float2 (&wt)[R], const float4* q
 }
 
 // ZIN (first pass only): slots EPT/2 + 1 .. EPT of every thread are zero.
-template <int LOGM, int THREADS, bool COLS, int P, bool ZIN = false>
+// PF: the first PF slots of a table (pfs, indexed by line position) are
+// loaded into pf one pass before the last, so their latency overlaps the
+// last exchange and butterflies (with one pass, at its start).
+template <int LOGM, int THREADS, bool COLS, int P, bool ZIN = false, int PF = 0>
 struct FftPass {
     using C = LineCfg<LOGM, THREADS>;
     static constexpr bool WAVE = !COLS && C::TPL <= 64;
@@ -429,8 +432,13 @@ struct FftPass {
     static constexpr bool ONE = C::template one_line<COLS>();
 
     __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, float* re, float* im, const float2* tw,
-                                               float4 (&w)[C::EPT / 2])
+                                               float4 (&w)[C::EPT / 2], const float2* pfs, float2 (&pf)[C::EPT])
     {
+        static_assert(PF <= C::EPT, "prefetch slots");
+        if constexpr (PF > 0 && C::NPASS == 1) {
+#pragma unroll
+            for (int i = 0; i < PF; ++i) pf[i] = pfs[t + i * C::TPL];
+        }
         if constexpr (P > 0) {
             exchange_sync<WAVE>();
             // layout of the previous pass (stride NS/16); the offset of slot i
@@ -468,6 +476,10 @@ struct FftPass {
         }
         if constexpr (!LAST) {
             if constexpr (tw_prefetch<LOGM>()) TwLoad<LOGM, THREADS, P + 1>::run(w, t, tw);
+            if constexpr (PF > 0 && P + 2 == C::NPASS) {
+#pragma unroll
+                for (int i = 0; i < PF; ++i) pf[i] = pfs[t + i * C::TPL];
+            }
             exchange_sync<WAVE>();
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -488,25 +500,33 @@ struct FftPass {
                     im[q] = v[b + NB * r].y;
                 }
             }
-            FftPass<LOGM, THREADS, COLS, P + 1>::run(v, t, re, im, tw, w);
+            FftPass<LOGM, THREADS, COLS, P + 1, false, PF>::run(v, t, re, im, tw, w, pfs, pf);
         }
     }
 };
 
 // ZIN: the upper half of the line is zero apart from position M/2, so slots
 // EPT/2 + 1 .. EPT of every thread hold zeros (the caller set them to
-// constant zero).
-template <int LOGM, int THREADS, bool COLS, bool ZIN = false>
+// constant zero).  PF > 0: pf[i] = pfs[t + i*TPL] for i < PF on return,
+// loaded during the FFT (pfs must be readable at every such position).
+template <int LOGM, int THREADS, bool COLS, bool ZIN = false, int PF = 0>
 struct FftTile {
     using C = LineCfg<LOGM, THREADS>;
     __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, int ll, float* lds, const float2* tw)
+    {
+        static_assert(PF == 0, "prefetch needs a destination");
+        float2 pf[C::EPT];
+        run(v, t, ll, lds, tw, nullptr, pf);
+    }
+    __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, int ll, float* lds, const float2* tw,
+                                               const float2* pfs, float2 (&pf)[C::EPT])
     {
         constexpr int LST = C::template lstride<COLS>();
         float* re = lds + ll * LST;
         float* im = lds + C::LPW * LST + ll * LST;
         float4 w[C::EPT / 2];
         if constexpr (tw_prefetch<LOGM>() && C::NPASS > 1) TwLoad<LOGM, THREADS, 1>::run(w, t, tw);
-        FftPass<LOGM, THREADS, COLS, 0, ZIN>::run(v, t, re, im, tw, w);
+        FftPass<LOGM, THREADS, COLS, 0, ZIN, PF>::run(v, t, re, im, tw, w, pfs, pf);
     }
 };
 
@@ -558,7 +578,12 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
         }
     }
 
-    FftTile<LOGM, THREADS, COLS, BLUE>::run(v, t, ll, lds, job.tw);
+    // Bluestein: the first half of the slots of D arrive during the first
+    // FFT (up to 4096 points; all of D, or D of 8192 points, spills at the
+    // 128-VGPR cap) and the live slots of the output chirp during the second
+    constexpr int PFD = (BLUE && C::NPASS <= 3) ? C::EPT / 2 : 0;
+    float2 pf[C::EPT];
+    FftTile<LOGM, THREADS, COLS, BLUE, PFD>::run(v, t, ll, lds, job.tw, job.dtab, pf);
 
     if constexpr (BLUE) {
         // forward conj(v*D), inverse conj(v)*D = conj(v*conj(D)); the outer
@@ -566,9 +591,11 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
             const uint32_t pos = uint32_t(t + i * C::TPL);
-            v[i] = cmul_conj(v[i], job.dtab[pos]);
+            v[i] = cmul_conj(v[i], i < PFD ? pf[i] : job.dtab[pos]);
         }
-        FftTile<LOGM, THREADS, COLS>::run(v, t, ll, lds, job.tw);
+        // the output chirp block is followed by readable entries up to
+        // position M/2 + M/16 (tables.hip), so every live slot can be loaded
+        FftTile<LOGM, THREADS, COLS, false, LIVE>::run(v, t, ll, lds, job.tw, job.chirp_out, pf);
     }
 
     if (valid) {
@@ -581,7 +608,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
                 float2 y;
                 if constexpr (BLUE) {
                     // forward conj(v)*c = conj(v*conj(c)), inverse conj(v*c)/n = conj(v*(c/n))
-                    y = cmul_conj(v[i], job.chirp_out[pos]);
+                    y = cmul_conj(v[i], pf[i]);
                 } else {
                     y = cscale(cxor(v[i], sgn), sc);
                 }

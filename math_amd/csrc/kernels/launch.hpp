@@ -59,8 +59,10 @@ hipError_t gen_h2_f32(float2* h2, const double2* chirp64, int64_t n, int64_t m, 
 
 hipError_t gen_twiddles(float2* table, hipStream_t stream);
 // Direction tables of a tile-resident Bluestein length (float2 entries):
-// [load fwd c | load inv conj(c) | D | conj(D) | store fwd conj(c) | store inv c/n]
-__host__ __device__ constexpr int64_t dir_tables_len(int64_t n, int64_t m) { return 4 * n + 2 * m; }
+// [load fwd c | load inv conj(c) | D | conj(D) | store fwd conj(c) | store inv c/n | m zeros]
+// The zero tail lets a tile read the store blocks at every position of its
+// live slots (up to M/2 + M/16 - 1 >= n) without a bounds check.
+__host__ __device__ constexpr int64_t dir_tables_len(int64_t n, int64_t m) { return 4 * n + 3 * m; }
 hipError_t gen_chirp(double2* chirp64, float2* chirp32, float2* dir, int n, int m, hipStream_t stream);
 hipError_t gen_dir_diagonal(float2* dir, const float2* dtab, int n, int m, hipStream_t stream);
 hipError_t gen_dtab(float2* dtab, const double2* chirp64, double* cos_scratch, int n, int m, hipStream_t stream);

@@ -63,6 +63,7 @@ __global__ void k_dir_diagonal(float2* dir, const float2* dtab, int n, int m)
     const float2 d = dtab[q];
     dir[2 * int64_t(n) + q] = d;
     dir[2 * int64_t(n) + m + q] = make_float2(d.x, -d.y);
+    dir[4 * int64_t(n) + 2 * int64_t(m) + q] = make_float2(0.0f, 0.0f);  // tail read past the store blocks
 }
 
 // cos(2*pi*q/m), q < m

@@ -594,7 +594,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
             v[i] = cmul_conj(v[i], i < PFD ? pf[i] : job.dtab[pos]);
         }
         // the output chirp block is followed by readable entries up to
-        // position M/2 + M/16 (tables.hip), so every live slot can be loaded
+        // position M/2 + M/16 (dir_tables_len), so every live slot can be loaded
         FftTile<LOGM, THREADS, COLS, false, LIVE>::run(v, t, ll, lds, job.tw, job.chirp_out, pf);
     }
 

@@ -396,6 +396,7 @@ def serving(M, torch, count, budget_gib, load):
     arrivals = np.cumsum(rng.exponential(1e6 / rate, count))
     lat, mk = server.replay(ws, hs, invs, srcs, arrivals)
     st = server.stats()
+    accuracy = served_accuracy(server, pool, sizes, invs, offs, srcs)
     server.close()
     del pool
     fl = sum(flops_of(w, h) for (w, h) in sizes)
@@ -415,9 +416,31 @@ def serving(M, torch, count, budget_gib, load):
         "micro_batches": st["launches"],
         "mean_batch": round(st["launched_requests"] / max(st["launches"], 1), 2),
         "plan_prepare_s": round(prep_s, 3),
+        "accuracy": accuracy,
         "workload": f"{count} requests, sides log-uniform in [32,4096], U[-1,1), 50/50 forward/inverse, "
                     "Poisson arrivals, inputs resident in HBM",
     }
+
+
+def served_accuracy(server, pool, sizes, invs, offs, srcs, count=8, max_points=1 << 21):
+    """Error of served results against numpy's FP64 FFT: `count` requests of the
+    stream (spread over its sizes, at most `max_points` each) submitted together,
+    so they run as one micro-batch, then fetched and compared."""
+    small = sorted((i for i, (w, h) in enumerate(sizes) if w * h <= max_points), key=lambda i: sizes[i][0] * sizes[i][1])
+    pick = small[:: max(1, len(small) // count)][:count]
+    tickets = [server.submit(sizes[i][0], sizes[i][1], invs[i], srcs[i]) for i in pick]
+    worst = {"nl2": 0.0, "nmax": 0.0, "tol_ratio": 0.0}
+    for i, tk in zip(pick, tickets):
+        w, h = sizes[i]
+        server.wait(tk)
+        y = server.fetch(tk, w * h).reshape(h, w)
+        server.release(tk)
+        x = pool[offs[i]:offs[i] + w * h].cpu().numpy().reshape(h, w)
+        m = _errors(y, _fft2_64(x, invs[i]))
+        worst["nl2"] = max(worst["nl2"], m["nl2"])
+        worst["nmax"] = max(worst["nmax"], m["nmax"])
+        worst["tol_ratio"] = max(worst["tol_ratio"], m["nl2"] / (2e-6 * max(1.0, math.log2(_padded(w) * _padded(h)))))
+    return {k: float("%.3e" % v) for k, v in worst.items()} | {"sampled": len(pick), "reference": "numpy fp64 fft"}
 
 
 def cpu_baseline(sizes, invs, seconds):

@@ -199,6 +199,9 @@ def main():
     }
 
     if not args.no_extras:
+        copy_gbs = device_copy_gbs(torch, dev)
+        out["roofline"]["copy_gbs"] = round(copy_gbs, 1)
+        out["roofline"]["frac_of_copy"] = round(gbs_kernel / copy_gbs, 4)
         out["accuracy"] = accuracy_sample(M, torch, src, dst, sizes, invs, offs)
         out["plain_path"] = plain_paths(M, torch, src, dst, sizes, invs, offs, flops)
         out["plain_path"]["speedup_batched_vs_device_plain"] = round(
@@ -223,6 +226,26 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sizes, invs, args.cpu_seconds)
 
     print(json.dumps(out), flush=True)
+
+
+def device_copy_gbs(torch, dev, nbytes=1 << 30, reps=10):
+    """Bandwidth of a plain 1 GiB device-to-device copy (bytes read + written per
+    second, event-timed): the HBM rate this box actually reaches."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(2):
+        b.copy_(a)
+    s0 = torch.cuda.Event(enable_timing=True)
+    s1 = torch.cuda.Event(enable_timing=True)
+    s0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    s1.record()
+    torch.cuda.synchronize()
+    ms = s0.elapsed_time(s1) / reps
+    del a, b
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
 def _padded(n):
@@ -463,9 +486,20 @@ def cpu_baseline(sizes, invs, seconds):
                 break
         passes += 1
     return {"value": round(done_flops / t_total / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"{done} transforms ({passes} pass(es) over the same 256-transform mixed batch), "
                       f"{t_total:.1f} s; FP32 restatement of the reference Bluestein/radix-2 pipeline "
                       f"(FFTW absent), threaded over lines"}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":

@@ -418,8 +418,9 @@ __device__ __forceinline__ void expand_twiddles(float2 (&wt)[R], const float4* q
 // ZIN (first pass only): slots EPT/2 + 1 .. EPT of every thread are zero.
 // PF: the first PF slots of a table (pfs, indexed by line position) are
 // loaded into pf one pass before the last, so their latency overlaps the
-// last exchange and butterflies (with one pass, at its start).
-template <int LOGM, int THREADS, bool COLS, int P, bool ZIN = false, int PF = 0>
+// last exchange and butterflies; slots PF .. PFL-1 at the start of the last
+// pass, after its exchange (with one pass, all PFL at its start).
+template <int LOGM, int THREADS, bool COLS, int P, bool ZIN = false, int PF = 0, int PFL = 0>
 struct FftPass {
     using C = LineCfg<LOGM, THREADS>;
     static constexpr bool WAVE = !COLS && C::TPL <= 64;
@@ -434,10 +435,10 @@ struct FftPass {
     __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, float* re, float* im, const float2* tw,
                                                float4 (&w)[C::EPT / 2], const float2* pfs, float2 (&pf)[C::EPT])
     {
-        static_assert(PF <= C::EPT, "prefetch slots");
+        static_assert(PF <= C::EPT && PFL <= C::EPT, "prefetch slots");
         if constexpr (PF > 0 && C::NPASS == 1) {
 #pragma unroll
-            for (int i = 0; i < PF; ++i) pf[i] = pfs[t + i * C::TPL];
+            for (int i = 0; i < (PFL > PF ? PFL : PF); ++i) pf[i] = pfs[t + i * C::TPL];
         }
         if constexpr (P > 0) {
             exchange_sync<WAVE>();
@@ -448,6 +449,10 @@ struct FftPass {
             for (int i = 0; i < C::EPT; ++i) {
                 const int q = q0 + xpad<NS / 16, ONE>(i * C::TPL);
                 v[i] = make_float2(re[q], im[q]);
+            }
+            if constexpr (LAST && PFL > PF) {
+#pragma unroll
+                for (int i = PF; i < PFL; ++i) pf[i] = pfs[t + i * C::TPL];
             }
         }
         // butterfly b reads and writes only the slots b + NB*r: update in place
@@ -500,21 +505,22 @@ struct FftPass {
                     im[q] = v[b + NB * r].y;
                 }
             }
-            FftPass<LOGM, THREADS, COLS, P + 1, false, PF>::run(v, t, re, im, tw, w, pfs, pf);
+            FftPass<LOGM, THREADS, COLS, P + 1, false, PF, PFL>::run(v, t, re, im, tw, w, pfs, pf);
         }
     }
 };
 
 // ZIN: the upper half of the line is zero apart from position M/2, so slots
 // EPT/2 + 1 .. EPT of every thread hold zeros (the caller set them to
-// constant zero).  PF > 0: pf[i] = pfs[t + i*TPL] for i < PF on return,
-// loaded during the FFT (pfs must be readable at every such position).
-template <int LOGM, int THREADS, bool COLS, bool ZIN = false, int PF = 0>
+// constant zero).  PF > 0: pf[i] = pfs[t + i*TPL] for i < max(PF, PFL) on
+// return, loaded during the FFT (pfs must be readable at every such
+// position).
+template <int LOGM, int THREADS, bool COLS, bool ZIN = false, int PF = 0, int PFL = 0>
 struct FftTile {
     using C = LineCfg<LOGM, THREADS>;
     __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, int ll, float* lds, const float2* tw)
     {
-        static_assert(PF == 0, "prefetch needs a destination");
+        static_assert(PF == 0 && PFL == 0, "prefetch needs a destination");
         float2 pf[C::EPT];
         run(v, t, ll, lds, tw, nullptr, pf);
     }
@@ -526,7 +532,7 @@ struct FftTile {
         float* im = lds + C::LPW * LST + ll * LST;
         float4 w[C::EPT / 2];
         if constexpr (tw_prefetch<LOGM>() && C::NPASS > 1) TwLoad<LOGM, THREADS, 1>::run(w, t, tw);
-        FftPass<LOGM, THREADS, COLS, 0, ZIN, PF>::run(v, t, re, im, tw, w, pfs, pf);
+        FftPass<LOGM, THREADS, COLS, 0, ZIN, PF, PFL>::run(v, t, re, im, tw, w, pfs, pf);
     }
 };
 
@@ -578,12 +584,14 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
         }
     }
 
-    // Bluestein: the first half of the slots of D arrive during the first
-    // FFT (up to 4096 points; all of D, or D of 8192 points, spills at the
-    // 128-VGPR cap) and the live slots of the output chirp during the second
+    // Bluestein: D arrives during the first FFT, its first half one pass
+    // before the end and the rest at the start of the last pass (up to 4096
+    // points; earlier loads of all of D, or D of 8192 points, spill at the
+    // 128-VGPR cap), and the live slots of the output chirp during the second
     constexpr int PFD = (BLUE && C::NPASS <= 3) ? C::EPT / 2 : 0;
+    constexpr int PFDL = PFD > 0 ? C::EPT : 0;
     float2 pf[C::EPT];
-    FftTile<LOGM, THREADS, COLS, BLUE, PFD>::run(v, t, ll, lds, job.tw, job.dtab, pf);
+    FftTile<LOGM, THREADS, COLS, BLUE, PFD, PFDL>::run(v, t, ll, lds, job.tw, job.dtab, pf);
 
     if constexpr (BLUE) {
         // forward conj(v*D), inverse conj(v)*D = conj(v*conj(D)); the outer
@@ -591,7 +599,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
             const uint32_t pos = uint32_t(t + i * C::TPL);
-            v[i] = cmul_conj(v[i], i < PFD ? pf[i] : job.dtab[pos]);
+            v[i] = cmul_conj(v[i], i < PFDL ? pf[i] : job.dtab[pos]);
         }
         // the output chirp block is followed by readable entries up to
         // position M/2 + M/16 (dir_tables_len), so every live slot can be loaded

@@ -1,5 +1,6 @@
 // C ABI of the single-transform path: plans, host and device exec, errors.
 #include "capi_util.hpp"
+#include "staging.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -17,6 +18,7 @@ struct mdft_plan_s {
     float2* scratch = nullptr;     // long-line passes only
     uint64_t scratch_elems = 0;
     hipStream_t stream = nullptr;  // private stream of mdft_exec
+    mdft::HostStaging staging;     // pinned ring of mdft_exec
 };
 
 namespace mdft {
@@ -78,6 +80,7 @@ int mdft_plan_destroy(mdft_plan plan)
         }
         if (plan->xd) (void)hipFree(plan->xd);
         if (plan->scratch) (void)hipFree(plan->scratch);
+        plan->staging.release();
     }
     delete plan;
     MDFT_API_END
@@ -108,10 +111,10 @@ int mdft_exec(mdft_plan plan, int inverse, void* data, size_t count)
         mdft::hip_check(hipStreamCreateWithFlags(&plan->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (!plan->xd) mdft::hip_check(hipMalloc(&plan->xd, bytes), "hipMalloc image");
     Context& ctx = Context::get(plan->device);
-    mdft::hip_check(hipMemcpyAsync(plan->xd, data, bytes, hipMemcpyHostToDevice, plan->stream), "copy in");
+    plan->staging.upload(plan->xd, data, bytes, plan->stream);
     ctx.enqueue_transform(*plan->row, *plan->col, (int)plan->w, (int)plan->h, inverse != 0, plan->xd, plan->xd,
                           plan->scratch, plan->stream);
-    mdft::hip_check(hipMemcpyAsync(data, plan->xd, bytes, hipMemcpyDeviceToHost, plan->stream), "copy out");
+    plan->staging.download(data, plan->xd, bytes, plan->stream);
     mdft::hip_check(hipStreamSynchronize(plan->stream), "exec");
     MDFT_API_END
 }

@@ -2,6 +2,7 @@
 // to back), the reference's Fft component (src/gpu/dft/fft.h:32-53,
 // fft.cpp:207-330) on this library's row kernels.
 #include "capi_util.hpp"
+#include "staging.hpp"
 
 #include <string>
 
@@ -16,6 +17,7 @@ struct mdft_fft_s {
     float2* xd = nullptr;                  // device copy for mdft_fft_exec, allocated on first use
     float2* scratch = nullptr;             // lines longer than one tile
     hipStream_t stream = nullptr;
+    mdft::HostStaging staging;             // pinned ring of mdft_fft_exec
 };
 
 extern "C" {
@@ -64,6 +66,7 @@ int mdft_fft_destroy(mdft_fft fft)
         }
         if (fft->xd) (void)hipFree(fft->xd);
         if (fft->scratch) (void)hipFree(fft->scratch);
+        fft->staging.release();
     }
     delete fft;
     MDFT_API_END
@@ -85,10 +88,10 @@ int mdft_fft_exec(mdft_fft fft, int inverse, void* data, size_t size)
     if (!fft->stream) mdft::hip_check(hipStreamCreateWithFlags(&fft->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (!fft->xd) mdft::hip_check(hipMalloc(&fft->xd, bytes), "hipMalloc data");
     Context& ctx = Context::get(fft->device);
-    mdft::hip_check(hipMemcpyAsync(fft->xd, data, bytes, hipMemcpyHostToDevice, fft->stream), "copy in");
+    fft->staging.upload(fft->xd, data, bytes, fft->stream);
     ctx.enqueue_axis(*fft->line, fft->xd, fft->xd, 1, fft->n, (int)fft->count, inverse != 0, false, fft->scratch,
                      fft->stream);
-    mdft::hip_check(hipMemcpyAsync(data, fft->xd, bytes, hipMemcpyDeviceToHost, fft->stream), "copy out");
+    fft->staging.download(data, fft->xd, bytes, fft->stream);
     mdft::hip_check(hipStreamSynchronize(fft->stream), "exec");
     MDFT_API_END
 }

@@ -66,14 +66,33 @@ typedef struct mdft_plan_s* mdft_plan;
  * above 2^27 + 1). */
 int mdft_plan_create(mdft_plan* plan, uint32_t width, uint32_t height);
 
-/* Free a plan and its device memory (ComputeVector destructor /
- * Dft::delete_buffers, dft.h:39, dft.cpp:311-324).  NULL is accepted. */
+/* Free a plan and the device memory it owns (its image copy and long-line
+ * scratch) (ComputeVector destructor / Dft::delete_buffers, dft.h:39,
+ * dft.cpp:311-324).  The per-length chirp, diagonal and twiddle tables are
+ * shared by every plan of the device through a table cache: they stay
+ * resident while another plan, batch or server uses the length, and
+ * afterwards until the cache exceeds its limit (mdft_cache_set_limit).  NULL
+ * is accepted. */
 int mdft_plan_destroy(mdft_plan plan);
 
-/* Padded lengths the plan uses per axis (compute_m, function.cpp:32-45; 1
- * when the axis is a power of two and needs no padding) and the plan's
- * device bytes. */
+/* Padded lengths the plan uses per axis: compute_m(side) (function.cpp:32-45),
+ * which is the side itself for a power of two; and the plan's device bytes
+ * (its share of the table cache counted in full, plus what it owns). */
 int mdft_plan_info(mdft_plan plan, uint32_t* m1, uint32_t* m2, uint64_t* device_bytes);
+
+/* ---- the per-device table cache ------------------------------------------- */
+
+/* Bytes of device tables cached for this thread's device (shared twiddles
+ * plus every cached length), the number of cached lengths, and the limit.
+ * No counterpart in the reference, whose Dft owns its tables per plan
+ * (src/gpu/dft/dft.cpp:262-309). */
+int mdft_cache_info(uint64_t* bytes, uint32_t* lengths, uint64_t* limit);
+
+/* Limit (bytes, default 4 GiB) of tables kept for lengths nothing uses: when
+ * a new length pushes the cache above it, the least recently requested
+ * unused lengths are freed.  Setting it frees the excess at once (0 frees
+ * every unused length).  Freeing device memory waits for the device. */
+int mdft_cache_set_limit(uint64_t bytes);
 
 /* In-place transform of count complex<float> values in host memory, blocking
  * until the result is back.  Replaces ComputeVector::exec(inverse, &vector)
@@ -86,7 +105,11 @@ int mdft_exec(mdft_plan plan, int inverse, void* data, size_t count);
 /* Enqueue the transform of a device image on a HIP stream (hipStream_t, NULL
  * for the default stream) without synchronising; src may equal dst.  The
  * analogue of recording Dft::compute_commands(command_buffer, inverse) into a
- * caller's command buffer (src/gpu/dft/dft.h:41, dft.cpp:326-347). */
+ * caller's command buffer (src/gpu/dft/dft.h:41, dft.cpp:326-347).  A plan
+ * with a side whose padded length exceeds 16384 runs through a scratch buffer
+ * it owns: a call on another stream than the plan's previous call waits on
+ * the device for that call first (calls recorded into a stream capture are
+ * not ordered this way: do not replay two such graphs of one plan at once). */
 int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, void* stream);
 
 /* ---- tile-shape tuning ---------------------------------------------------- */
@@ -209,6 +232,19 @@ int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count,
  * pointers may be captured into a caller's graph. */
 int mdft_batch_exec_device(mdft_batch batch, const void* src, void* dst, void* stream);
 
+/* Enqueue one launch phase of a grid-shape batch (the default shape, with or
+ * without MDFT_BATCH_GRAPH): phase 1 runs whole small images and row tiles,
+ * phase 2 the column tiles and must follow a phase 1 call with the same
+ * pointers.  For callers that interleave other work between the phases, and
+ * for timing each phase.  MDFT_ERR_STATE for persistent shapes. */
+int mdft_batch_exec_phase(mdft_batch batch, int phase, const void* src, void* dst, void* stream);
+
+/* Work items of each phase, their algorithmic bytes (16 bytes per point per
+ * axis pass: one read and one write of the image) and the workgroup sizes in
+ * threads of each phase's tiles.  Any pointer may be NULL. */
+int mdft_batch_info(mdft_batch batch, uint32_t* items1, uint32_t* items2, uint64_t* bytes1, uint64_t* bytes2,
+                    uint32_t* threads1, uint32_t* threads2);
+
 /* Device error flag of the last completed launch: 0, or MDFT_ERR_TIMEOUT if
  * a dependency wait expired.  Synchronises the batch's last stream. */
 int mdft_batch_check(mdft_batch batch);
@@ -219,7 +255,8 @@ int mdft_batch_destroy(mdft_batch batch);
 /* No counterpart in the reference (its exec blocks per call,
  * src/gpu/dft/compute.cpp:257-283).  The server owns `hbm_budget_bytes` of
  * device memory; each request needs a result slot of width*height*8 bytes
- * and is admitted in arrival order when its slot fits.  A scheduler thread
+ * and is admitted in arrival order when its slot, the slots in use and the
+ * resident line-plan tables fit the budget.  A scheduler thread
  * launches the admitted requests as micro-batches on `streams` streams (at
  * most `max_batch` requests each), so requests arriving while others run
  * join the next launch and overlap with them.  submit/wait/release/stats are
@@ -235,6 +272,7 @@ typedef struct {
     uint64_t launched_requests; /* requests over all micro-batches */
     uint64_t budget_bytes;
     uint64_t used_bytes;        /* slots currently held */
+    uint64_t table_bytes;       /* resident line-plan tables (counted against the budget) */
 } mdft_server_counters;
 
 typedef struct {
@@ -253,11 +291,22 @@ int mdft_server_create(mdft_server* server, uint64_t hbm_budget_bytes, uint32_t 
 int mdft_server_prepare(mdft_server server, const uint32_t* lengths, uint32_t count);
 
 /* Queue a request.  src is a device pointer to width*height complex<float>
- * that must stay valid until the request completes.  MDFT_ERR_BUDGET when
- * the slot is larger than the whole budget; MDFT_ERR_SIZE beyond the batch
+ * that must be fully written when submit is called (work that produces it on
+ * a stream must have completed: synchronise first, or use
+ * mdft_server_submit_stream) and stay valid until the request completes.
+ * The line plans of new lengths are built on the calling thread and stay
+ * resident for the server's lifetime; their bytes count against the budget
+ * together with the result slots.  MDFT_ERR_BUDGET when the slot plus the
+ * resident tables exceed the whole budget; MDFT_ERR_SIZE beyond the batch
  * padded-length limit (8192). */
 int mdft_server_submit(mdft_server server, uint32_t width, uint32_t height, int inverse, const void* src,
                        uint64_t* ticket);
+
+/* As mdft_server_submit, for an input still being produced on `stream`
+ * (hipStream_t, NULL for the default stream): the request's launch waits on
+ * the device for the work enqueued on that stream before this call. */
+int mdft_server_submit_stream(mdft_server server, uint32_t width, uint32_t height, int inverse, const void* src,
+                              void* stream, uint64_t* ticket);
 
 /* Wait for a request (timeout_us < 0: no limit; MDFT_ERR_TIMEOUT when it
  * expires).  result receives the device address of the output (valid until

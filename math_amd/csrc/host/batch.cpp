@@ -37,9 +37,27 @@ struct mdft_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     uint32_t spin_limit = 1u << 20;  // fail-safe of the dependency wait: then the error flag
+    uint64_t phase_bytes[2] = {0, 0};  // algorithmic bytes (16 per point and pass) per launch phase
 };
 
 namespace {
+
+// Algorithmic bytes of each launch phase: every axis pass reads and writes
+// its image once (16 bytes per point); a job belongs to the phase of its items.
+void count_phase_bytes(mdft_batch_s* b)
+{
+    const auto& items = b->build.items;
+    std::vector<int> phase(b->build.jobs.size(), 0);
+    for (size_t i = 0; i < items.size(); ++i) {
+        const int ph = i < b->nphase1 ? 1 : 2;
+        phase[items[i].x] = ph;
+        if (items[i].y & mdft::BATCH_WHOLE) phase[items[i].y & ~mdft::BATCH_WHOLE] = ph;
+    }
+    for (size_t j = 0; j < phase.size(); ++j) {
+        const mdft_desc& d = b->descs[b->job_image[j]];
+        if (phase[j]) b->phase_bytes[phase[j] - 1] += 16ull * d.width * d.height;
+    }
+}
 
 void enqueue_batch(mdft_batch_s* b, hipStream_t stream)
 {
@@ -176,6 +194,7 @@ int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count,
     b->nphase1 = b->build.nphase1;
     b->nitems = (uint32_t)b->build.items.size();
     b->nctl = mdft::BATCH_CTL_HEADER + b->build.ncounters;
+    count_phase_bytes(b.get());
 
     int per_cu = 0;
     mdft::hip_check(mdft::batch_occupancy(T, &per_cu), "occupancy");
@@ -204,13 +223,14 @@ int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count,
     MDFT_API_END
 }
 
-int mdft_batch_exec_device(mdft_batch b, const void* src, void* dst, void* stream_p)
+}  // extern "C"
+
+namespace {
+
+// Point the job table at src/dst (re-uploaded when they changed) and copy the
+// 1x1 identities.
+void prepare_exec(mdft_batch b, const void* src, void* dst, hipStream_t stream)
 {
-    MDFT_API_BEGIN
-    if (!b) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
-    if (!src || !dst) throw Error(MDFT_ERR_DATA_SIZE, "Wrong DFT buffer size");
-    mdft::DeviceGuard g(b->device);
-    hipStream_t stream = static_cast<hipStream_t>(stream_p);
     if (src != b->cur_src || dst != b->cur_dst) {
         // the pinned staging copy of an earlier patch may still be in flight
         if (b->cur_src) mdft::hip_check(hipDeviceSynchronize(), "synchronize before re-patch");
@@ -241,11 +261,64 @@ int mdft_batch_exec_device(mdft_batch b, const void* src, void* dst, void* strea
                                            hipMemcpyDeviceToDevice, stream),
                             "identity copy");
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdft_batch_exec_device(mdft_batch b, const void* src, void* dst, void* stream_p)
+{
+    MDFT_API_BEGIN
+    if (!b) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
+    if (!src || !dst) throw Error(MDFT_ERR_DATA_SIZE, "Wrong DFT buffer size");
+    mdft::DeviceGuard g(b->device);
+    hipStream_t stream = static_cast<hipStream_t>(stream_p);
+    prepare_exec(b, src, dst, stream);
     if (b->gexec)
         mdft::hip_check(hipGraphLaunch(b->gexec, stream), "graph launch");
     else
         enqueue_batch(b, stream);
     b->last_stream = stream;
+    MDFT_API_END
+}
+
+int mdft_batch_exec_phase(mdft_batch b, int phase, const void* src, void* dst, void* stream_p)
+{
+    MDFT_API_BEGIN
+    if (!b) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
+    if (!src || !dst) throw Error(MDFT_ERR_DATA_SIZE, "Wrong DFT buffer size");
+    if (phase != 1 && phase != 2) throw Error(MDFT_ERR_INVALID, "mdft_batch_exec_phase: phase must be 1 or 2");
+    if (b->flags & (MDFT_BATCH_PERSISTENT | MDFT_BATCH_DEPENDENT))
+        throw Error(MDFT_ERR_STATE, "mdft_batch_exec_phase: only grid-shape batches run by phase");
+    mdft::DeviceGuard g(b->device);
+    hipStream_t stream = static_cast<hipStream_t>(stream_p);
+    if (phase == 1) {
+        prepare_exec(b, src, dst, stream);
+        mdft::hip_check(mdft::launch_batch_grid(b->threads, b->jobs_dev, b->items_dev, b->nphase1, stream),
+                        "batch launch (phase 1)");
+    } else {
+        if (src != b->cur_src || dst != b->cur_dst)
+            throw Error(MDFT_ERR_STATE, "mdft_batch_exec_phase: phase 2 needs the pointers of the phase 1 call");
+        mdft::hip_check(mdft::launch_batch_grid(b->threads_cols, b->jobs_dev, b->items_dev + b->nphase1,
+                                                b->nitems - b->nphase1, stream),
+                        "batch launch (phase 2)");
+    }
+    b->last_stream = stream;
+    MDFT_API_END
+}
+
+int mdft_batch_info(mdft_batch b, uint32_t* items1, uint32_t* items2, uint64_t* bytes1, uint64_t* bytes2,
+                    uint32_t* threads1, uint32_t* threads2)
+{
+    MDFT_API_BEGIN
+    if (!b) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
+    if (items1) *items1 = b->nphase1;
+    if (items2) *items2 = b->nitems - b->nphase1;
+    if (bytes1) *bytes1 = b->phase_bytes[0];
+    if (bytes2) *bytes2 = b->phase_bytes[1];
+    if (threads1) *threads1 = (uint32_t)b->threads;
+    if (threads2) *threads2 = (uint32_t)b->threads_cols;
     MDFT_API_END
 }
 

@@ -33,7 +33,8 @@ int batch_threads_for(const uint32_t* widths, const uint32_t* heights, size_t co
                                            std::to_string(1 << limit));
         maxlogm = std::max({maxlogm, l1, l2});
     }
-    return maxlogm > batch_max_logm(256) ? 1024 : 256;
+    // the smallest tile that holds a whole line of every length
+    return maxlogm <= batch_max_logm(256) ? 256 : maxlogm <= batch_max_logm(512) ? 512 : 1024;
 }
 
 void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, int TC, bool dependent,

@@ -29,9 +29,9 @@ struct BatchEntry {
     float2* dst;
 };
 
-// Workgroup size a set of sizes needs for whole images and row tiles (256 up
-// to padded length 4096, 1024 up to 8192); throws MDFT_ERR_SIZE beyond the
-// batch limit.
+// Workgroup size a set of sizes needs for whole images and row tiles: the
+// smallest of 256, 512 and 1024 threads that holds one line of the longest
+// padded length; throws MDFT_ERR_SIZE beyond the batch limit (8192).
 int batch_threads_for(const uint32_t* widths, const uint32_t* heights, size_t count);
 
 // Builds jobs and items.  dependent: one-launch ordering with per-image

@@ -19,6 +19,7 @@ struct mdft_plan_s {
     uint64_t scratch_elems = 0;
     hipStream_t stream = nullptr;  // private stream of mdft_exec
     mdft::HostStaging staging;     // pinned ring of mdft_exec
+    mdft::ScratchOrder order;      // calls sharing `scratch` on different streams
 };
 
 namespace mdft {
@@ -80,6 +81,7 @@ int mdft_plan_destroy(mdft_plan plan)
         }
         if (plan->xd) (void)hipFree(plan->xd);
         if (plan->scratch) (void)hipFree(plan->scratch);
+        plan->order.release();
         plan->staging.release();
     }
     delete plan;
@@ -112,8 +114,10 @@ int mdft_exec(mdft_plan plan, int inverse, void* data, size_t count)
     if (!plan->xd) mdft::hip_check(hipMalloc(&plan->xd, bytes), "hipMalloc image");
     Context& ctx = Context::get(plan->device);
     plan->staging.upload(plan->xd, data, bytes, plan->stream);
+    if (plan->scratch) plan->order.before(plan->stream);
     ctx.enqueue_transform(*plan->row, *plan->col, (int)plan->w, (int)plan->h, inverse != 0, plan->xd, plan->xd,
                           plan->scratch, plan->stream);
+    if (plan->scratch) plan->order.after(plan->stream);
     plan->staging.download(data, plan->xd, bytes, plan->stream);
     mdft::hip_check(hipStreamSynchronize(plan->stream), "exec");
     MDFT_API_END
@@ -126,9 +130,28 @@ int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, vo
     if (!src || !dst) throw Error(MDFT_ERR_DATA_SIZE, "Wrong DFT buffer size");
     mdft::DeviceGuard g(plan->device);
     Context& ctx = Context::get(plan->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (plan->scratch) plan->order.before(s);
     ctx.enqueue_transform(*plan->row, *plan->col, (int)plan->w, (int)plan->h, inverse != 0,
-                          static_cast<const float2*>(src), static_cast<float2*>(dst), plan->scratch,
-                          static_cast<hipStream_t>(stream));
+                          static_cast<const float2*>(src), static_cast<float2*>(dst), plan->scratch, s);
+    if (plan->scratch) plan->order.after(s);
+    MDFT_API_END
+}
+
+int mdft_cache_info(uint64_t* bytes, uint32_t* lengths, uint64_t* limit)
+{
+    MDFT_API_BEGIN
+    Context& ctx = Context::get(Context::current_device());
+    if (bytes) *bytes = ctx.cached_bytes();
+    if (lengths) *lengths = ctx.cached_lengths();
+    if (limit) *limit = ctx.cache_limit();
+    MDFT_API_END
+}
+
+int mdft_cache_set_limit(uint64_t bytes)
+{
+    MDFT_API_BEGIN
+    Context::get(Context::current_device()).set_cache_limit(bytes);
     MDFT_API_END
 }
 

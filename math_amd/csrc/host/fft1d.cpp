@@ -18,6 +18,7 @@ struct mdft_fft_s {
     float2* scratch = nullptr;             // lines longer than one tile
     hipStream_t stream = nullptr;
     mdft::HostStaging staging;             // pinned ring of mdft_fft_exec
+    mdft::ScratchOrder order;              // calls sharing `scratch` on different streams
 };
 
 extern "C" {
@@ -66,6 +67,7 @@ int mdft_fft_destroy(mdft_fft fft)
         }
         if (fft->xd) (void)hipFree(fft->xd);
         if (fft->scratch) (void)hipFree(fft->scratch);
+        fft->order.release();
         fft->staging.release();
     }
     delete fft;
@@ -89,8 +91,10 @@ int mdft_fft_exec(mdft_fft fft, int inverse, void* data, size_t size)
     if (!fft->xd) mdft::hip_check(hipMalloc(&fft->xd, bytes), "hipMalloc data");
     Context& ctx = Context::get(fft->device);
     fft->staging.upload(fft->xd, data, bytes, fft->stream);
+    if (fft->scratch) fft->order.before(fft->stream);
     ctx.enqueue_axis(*fft->line, fft->xd, fft->xd, 1, fft->n, (int)fft->count, inverse != 0, false, fft->scratch,
                      fft->stream);
+    if (fft->scratch) fft->order.after(fft->stream);
     fft->staging.download(data, fft->xd, bytes, fft->stream);
     mdft::hip_check(hipStreamSynchronize(fft->stream), "exec");
     MDFT_API_END
@@ -110,9 +114,11 @@ int mdft_fft_exec_device(mdft_fft fft, int inverse, const void* src, void* dst, 
             mdft::hip_check(hipMemcpyAsync(dst, src, sizeof(float2) * need, hipMemcpyDeviceToDevice, s), "copy");
         return MDFT_OK;
     }
+    if (fft->scratch) fft->order.before(s);
     Context::get(fft->device)
         .enqueue_axis(*fft->line, static_cast<const float2*>(src), static_cast<float2*>(dst), 1, fft->n,
                       (int)fft->count, inverse != 0, false, fft->scratch, s);
+    if (fft->scratch) fft->order.after(s);
     MDFT_API_END
 }
 

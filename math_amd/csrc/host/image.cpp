@@ -18,6 +18,7 @@ struct mdft_image_s {
     float2* buf = nullptr;  // complex image between the passes (the reference's x_d)
     float2* scratch = nullptr;
     uint64_t scratch_elems = 0;
+    mdft::ScratchOrder order;  // calls sharing buf and scratch on different streams
 };
 
 namespace {
@@ -64,6 +65,7 @@ int mdft_image_destroy(mdft_image image)
         (void)hipDeviceSynchronize();
         if (image->buf) (void)hipFree(image->buf);
         if (image->scratch) (void)hipFree(image->scratch);
+        image->order.release();
     }
     delete image;
     MDFT_API_END
@@ -96,6 +98,7 @@ int mdft_image_exec_device(mdft_image image, const void* src, uint32_t src_width
     const LinePlan& row = *image->row;
     const LinePlan& col = *image->col;
     const bool fused = !row.large() && !col.large() && (w > 1 || h > 1);
+    image->order.before(s);
     if (!fused) {
         // luminance -> complex buffer, ordinary transform in place, magnitude
         io.iel = 1, io.ils = (int64_t)src_pitch;
@@ -103,6 +106,7 @@ int mdft_image_exec_device(mdft_image image, const void* src, uint32_t src_width
         launch(mdft::launch_image_luminance(io, image->buf, w, h, s), "image luminance launch");
         ctx.enqueue_transform(row, col, w, h, false, image->buf, image->buf, image->scratch, s);
         launch(mdft::launch_image_magnitude(io, image->buf, w, h, s), "image magnitude launch");
+        image->order.after(s);
         return MDFT_OK;
     }
     if (w > 1) {
@@ -122,6 +126,7 @@ int mdft_image_exec_device(mdft_image image, const void* src, uint32_t src_width
         const bool wide = ctx.wide_tiles(col.logm, col.blue, true, w);
         launch(mdft::launch_image_axis(col.logm, j, io, col.blue, true, mode, wide, s), "image column pass launch");
     }
+    image->order.after(s);
     MDFT_API_END
 }
 

@@ -10,6 +10,8 @@
 
 namespace mdft {
 
+constexpr int64_t LineMaxTile = int64_t(1) << MAX_LOGM;
+
 void hip_check(hipError_t e, const char* what)
 {
     if (e != hipSuccess) throw Error(MDFT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
@@ -56,6 +58,35 @@ DeviceGuard::~DeviceGuard()
     if (hipGetDevice(&cur) == hipSuccess && cur != prev_) (void)hipSetDevice(prev_);
 }
 
+bool ScratchOrder::capturing(hipStream_t s) const
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(s, &st), "hipStreamIsCapturing");
+    return st != hipStreamCaptureStatusNone;
+}
+
+void ScratchOrder::before(hipStream_t s)
+{
+    if (!pending_ || s == last_ || capturing(s)) return;
+    hip_check(hipStreamWaitEvent(s, ev_, 0), "hipStreamWaitEvent");
+}
+
+void ScratchOrder::after(hipStream_t s)
+{
+    if (capturing(s)) return;
+    if (!ev_) hip_check(hipEventCreateWithFlags(&ev_, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(ev_, s), "hipEventRecord");
+    last_ = s;
+    pending_ = true;
+}
+
+void ScratchOrder::release()
+{
+    if (ev_) (void)hipEventDestroy(ev_);
+    ev_ = nullptr;
+    pending_ = false;
+}
+
 namespace {
 thread_local int t_device = -1;
 std::mutex g_ctx_mu;
@@ -63,6 +94,13 @@ std::unordered_map<int, Context*> g_ctx;
 }  // namespace
 
 int Context::current_device() { return t_device >= 0 ? t_device : 0; }
+
+Context* Context::find(int device)
+{
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    auto it = g_ctx.find(device);
+    return it == g_ctx.end() ? nullptr : it->second;
+}
 
 void Context::set_thread_device(int device) { t_device = device; }
 
@@ -90,6 +128,9 @@ Context::Context(int device) : device_(device)
     hip_check(hipMalloc(&twiddles_, sizeof(float2) * (size_t)tw_total()), "hipMalloc twiddles");
     hip_check(gen_twiddles(twiddles_, gen_stream_), "twiddle generation");
     hip_check(hipStreamSynchronize(gen_stream_), "twiddle generation");
+    // FP64 table scratch for every tile-resident Bluestein length (2n + m doubles)
+    scratch64_doubles_ = 2 * (size_t(LineMaxTile) / 2 + 1) + size_t(LineMaxTile);
+    hip_check(hipMalloc(&scratch64_, sizeof(double) * scratch64_doubles_), "hipMalloc scratch");
     // default tile shapes: wide tiles for columns of 1024 points and more
     for (int l = 0; l <= MAX_LOGM; ++l)
         for (int b = 0; b < 2; ++b) variant_[l][b][1] = (l >= 10) ? 1 : 0;
@@ -100,7 +141,10 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
     const int m = compute_m(n);  // validates n
     std::lock_guard<std::mutex> lk(mu_);
     auto it = lines_.find(n);
-    if (it != lines_.end()) return it->second;
+    if (it != lines_.end()) {
+        it->second.last_use = ++use_clock_;
+        return it->second.plan;
+    }
 
     auto lp = std::make_shared<LinePlan>();
     lp->n = n;
@@ -122,11 +166,14 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
         hip_check(gen_large_twiddles(lp->tw_lo, lp->tw_hi, m, gen_stream_), "twiddle generation");
     }
     if (lp->blue) {
+        // the FP64 scratch is sized at context creation for every tile-resident
+        // length; only long lines grow it (hipFree waits for the device)
         const size_t need = 2 * (size_t)n + (size_t)m;
         if (need > scratch64_doubles_) {
             hip_check(hipStreamSynchronize(gen_stream_), "table generation");
             if (scratch64_) hip_check(hipFree(scratch64_), "hipFree");
             scratch64_ = nullptr;
+            scratch64_doubles_ = 0;
             hip_check(hipMalloc(&scratch64_, sizeof(double) * need), "hipMalloc scratch");
             scratch64_doubles_ = need;
         }
@@ -148,34 +195,83 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
             // long lines: D = FFT_M(h2)/M by this library's four-step FFT on the
             // FP32-rounded embedding (the reference computes its D with an FP32 FFT
             // as well, dft.cpp:57-69)
-            float2* tmp = nullptr;
-            hip_check(hipMalloc(&tmp, sizeof(float2) * 2 * (size_t)m), "hipMalloc diagonal scratch");
-            hip_check(gen_h2_f32(tmp, chirp64, n, m, gen_stream_), "embedding");
-            LinePlan pw;  // the power-of-two FFT of length m, sharing this plan's twiddles
-            pw.n = m;
-            pw.m = m;
-            pw.logm = lp->logm;
-            pw.logm1 = lp->logm1;
-            pw.logm2 = lp->logm2;
-            pw.tw_lo = lp->tw_lo;
-            pw.tw_hi = lp->tw_hi;
-            enqueue_large_fft(pw, tmp, lp->dtab, 1, m, 1, 1.0f / (float)m, tmp + m, gen_stream_);
-            pw.tw_lo = pw.tw_hi = nullptr;  // owned by lp
+            struct DevFree {
+                void operator()(float2* p) const { (void)hipFree(p); }
+            };
+            float2* raw = nullptr;
+            hip_check(hipMalloc(&raw, sizeof(float2) * 2 * (size_t)m), "hipMalloc diagonal scratch");
+            std::unique_ptr<float2, DevFree> tmp(raw);
+            hip_check(gen_h2_f32(tmp.get(), chirp64, n, m, gen_stream_), "embedding");
+            LargeFft f;
+            f.m = m;
+            f.logm1 = lp->logm1;
+            f.logm2 = lp->logm2;
+            f.tw_lo = lp->tw_lo;
+            f.tw_hi = lp->tw_hi;
+            enqueue_large_fft(f, tmp.get(), lp->dtab, 1, m, 1, 1.0f / (float)m, tmp.get() + m, gen_stream_);
             hip_check(hipStreamSynchronize(gen_stream_), "diagonal generation");
-            hip_check(hipFree(tmp), "hipFree");
         }
     }
     hip_check(hipStreamSynchronize(gen_stream_), "table generation");
-    lines_[n] = lp;
+    lines_[n] = CacheEntry{lp, ++use_clock_};
+    trim_locked(cache_limit_);
     return lp;
+}
+
+std::shared_ptr<const LinePlan> Context::find_line_plan(int n)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = lines_.find(n);
+    if (it == lines_.end()) return nullptr;
+    it->second.last_use = ++use_clock_;
+    return it->second.plan;
+}
+
+// Free the least recently requested lengths that no plan, batch or server
+// holds until the cached bytes are within limit (hipFree waits for the device,
+// so no launch that still reads a table is affected).
+void Context::trim_locked(uint64_t limit)
+{
+    uint64_t total = 0;
+    for (auto& kv : lines_) total += kv.second.plan->bytes;
+    while (total > limit) {
+        auto victim = lines_.end();
+        for (auto it = lines_.begin(); it != lines_.end(); ++it)
+            if (it->second.plan.use_count() == 1 &&
+                (victim == lines_.end() || it->second.last_use < victim->second.last_use))
+                victim = it;
+        if (victim == lines_.end()) break;  // everything left is in use
+        total -= victim->second.plan->bytes;
+        lines_.erase(victim);
+    }
 }
 
 uint64_t Context::cached_bytes()
 {
     std::lock_guard<std::mutex> lk(mu_);
     uint64_t s = sizeof(float2) * (uint64_t)tw_total();
-    for (auto& kv : lines_) s += kv.second->bytes;
+    for (auto& kv : lines_) s += kv.second.plan->bytes;
     return s;
+}
+
+uint32_t Context::cached_lengths()
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    return (uint32_t)lines_.size();
+}
+
+uint64_t Context::cache_limit()
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    return cache_limit_;
+}
+
+void Context::set_cache_limit(uint64_t bytes)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    cache_limit_ = bytes;
+    DeviceGuard g(device_);
+    trim_locked(bytes);
 }
 
 AxisJob Context::make_job(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
@@ -206,10 +302,10 @@ static void large_step(int sublogm, const float2* tw, LargeJob job, hipStream_t 
     hip_check(launch_large_step(sublogm, job, stream), "long-line step launch");
 }
 
-void Context::enqueue_large_fft(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
+void Context::enqueue_large_fft(const LargeFft& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
                                 int64_t count, float scale, float2* scratch, hipStream_t stream) const
 {
-    // forward FFT (power of two, n == m) of count lines: A then C
+    // forward FFT (power of two) of count lines: A then C
     const int64_t M = lp.m, M1 = int64_t(1) << lp.logm1, M2 = int64_t(1) << lp.logm2;
     LargeJob a{};
     a.src = src;

@@ -56,9 +56,20 @@ struct LinePlan {
     ~LinePlan();
 };
 
+// Geometry and twiddles of a power-of-two four-step FFT of length m (a view:
+// the tables belong to a LinePlan).
+struct LargeFft {
+    int64_t m = 0;
+    int logm1 = 0, logm2 = 0;
+    const float2* tw_lo = nullptr;
+    const float2* tw_hi = nullptr;
+};
+
 class Context {
 public:
     static Context& get(int device);
+    // the device's context if one was created, else null (creates nothing)
+    static Context* find(int device);
     static int current_device();
     static void set_thread_device(int device);
 
@@ -66,9 +77,17 @@ public:
     int cu_count() const { return cu_count_; }
     const float2* twiddles(int logm) const { return twiddles_ + tw_off(logm); }
 
-    // Cached, device-resident plan of one line length (thread-safe).
+    // Cached, device-resident plan of one line length (thread-safe).  The
+    // cache holds at most cache_limit() bytes of tables that no caller holds:
+    // beyond it the least recently requested unused lengths are freed.
     std::shared_ptr<const LinePlan> line_plan(int n);
+    // the cached plan of length n, or null; builds and allocates nothing
+    std::shared_ptr<const LinePlan> find_line_plan(int n);
     uint64_t cached_bytes();
+    uint32_t cached_lengths();
+    uint64_t cache_limit();
+    // set the limit and free unused tables beyond it at once
+    void set_cache_limit(uint64_t bytes);
 
     // Enqueue one axis pass of one image (scratch: lines longer than one tile).
     void enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
@@ -81,10 +100,10 @@ public:
     // dst, both natural order), the building block of the long-line path.
     void enqueue_large_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
                             int64_t count, bool inverse, float2* scratch, hipStream_t stream) const;
-    // Forward power-of-two four-step FFT of length lp.m, times scale
+    // Forward power-of-two four-step FFT of length f.m, times scale
     // (scratch: count * m elements).
-    void enqueue_large_fft(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
-                           int64_t count, float scale, float2* scratch, hipStream_t stream) const;
+    void enqueue_large_fft(const LargeFft& f, const float2* src, float2* dst, int64_t es, int64_t ls, int64_t count,
+                           float scale, float2* scratch, hipStream_t stream) const;
 
     AxisJob make_job(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,
                      bool inverse) const;
@@ -107,8 +126,34 @@ private:
     double* scratch64_ = nullptr;
     size_t scratch64_doubles_ = 0;
     std::mutex mu_;
-    std::unordered_map<int, std::shared_ptr<const LinePlan>> lines_;
+    struct CacheEntry {
+        std::shared_ptr<const LinePlan> plan;
+        uint64_t last_use = 0;
+    };
+    std::unordered_map<int, CacheEntry> lines_;
+    uint64_t use_clock_ = 0;
+    uint64_t cache_limit_ = uint64_t(4) << 30;
+    void trim_locked(uint64_t limit);
     uint8_t variant_[MAX_LOGM + 1][2][2] = {};
+};
+
+
+// Orders the calls that share one scratch buffer (long-line plans) when a
+// caller enqueues them on different streams: a call on another stream than
+// the previous one first waits for the previous call's completion event.
+// Calls recorded into a stream capture are not tracked (their replays share
+// the buffer; a caller must not replay two of them concurrently).
+class ScratchOrder {
+public:
+    void before(hipStream_t s);
+    void after(hipStream_t s);
+    void release();
+
+private:
+    bool capturing(hipStream_t s) const;
+    hipEvent_t ev_ = nullptr;
+    hipStream_t last_ = nullptr;
+    bool pending_ = false;
 };
 
 // RAII guard that makes a device current for the calling thread.

@@ -103,6 +103,7 @@ struct Request {
     uint32_t w = 0, h = 0;
     bool inverse = false;
     const float2* src = nullptr;
+    hipEvent_t ready = nullptr;  // producer event of submit_stream (null: src already written)
     uint64_t bytes = 0, off = 0;
     int state = PENDING;
     double t_submit = 0, t_launch = 0, t_done = 0;
@@ -131,6 +132,11 @@ struct mdft_server_s {
     std::condition_variable cv_work, cv_done;
     std::deque<uint64_t> pending;
     std::unordered_map<uint64_t, Request> reqs;
+    // line plans of every length the server has seen: resident for its
+    // lifetime and counted against the budget with the result slots
+    std::unordered_map<int, std::shared_ptr<const mdft::LinePlan>> plans;
+    uint64_t table_bytes = 0;
+    std::vector<hipEvent_t> free_events;
     uint64_t next_id = 1;
     bool stop = false;
     std::thread sched;
@@ -142,6 +148,36 @@ struct mdft_server_s {
 namespace {
 
 size_t table_bytes(size_t jobs, size_t items) { return sizeof(BatchJob) * jobs + sizeof(uint2) * items; }
+
+uint64_t slot_bytes(uint64_t bytes) { return (bytes + 255) & ~uint64_t(255); }
+
+// Makes the plans of a request's lengths resident (built on the calling
+// thread, not the scheduler's) and returns the table bytes they add.
+uint64_t hold_plans(mdft_server_s* s, const uint32_t* lengths, size_t count,
+                    std::vector<std::shared_ptr<const mdft::LinePlan>>& out)
+{
+    Context& ctx = Context::get(s->device);
+    uint64_t add = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const int n = (int)lengths[i];
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->plans.count(n)) continue;
+        }
+        bool dup = false;
+        for (auto& p : out) dup |= p->n == n;
+        if (dup) continue;
+        out.push_back(ctx.line_plan(n));
+        add += out.back()->bytes;
+    }
+    return add;
+}
+
+void commit_plans(mdft_server_s* s, std::vector<std::shared_ptr<const mdft::LinePlan>>& held)
+{
+    for (auto& p : held)
+        if (s->plans.emplace(p->n, p).second) s->table_bytes += p->bytes;
+}
 
 void grow(Lane& L, size_t njobs, size_t nitems)
 {
@@ -228,6 +264,7 @@ void scheduler(mdft_server_s* s)
                     break;
                 }
             std::vector<mdft::BatchEntry> entries;
+            std::vector<hipEvent_t> waits;
             if (free_lane) {
                 std::lock_guard<std::mutex> lk(s->mu);
                 if (s->stop && s->pending.empty()) {
@@ -238,17 +275,29 @@ void scheduler(mdft_server_s* s)
                 while (!s->pending.empty() && entries.size() < s->max_batch) {
                     Request& r = s->reqs[s->pending.front()];
                     uint64_t off = 0;
+                    // slots in use + this slot + resident tables within the budget
+                    if (s->arena.used() + slot_bytes(r.bytes) + s->table_bytes > s->arena.size()) break;
                     if (!s->arena.alloc(r.bytes, &off)) break;  // head of line waits for memory
                     r.off = off;
                     r.state = RUNNING;
                     r.t_launch = now_us();
                     float2* slot = reinterpret_cast<float2*>(s->arena.base() + off);
                     entries.push_back({r.w, r.h, r.inverse, r.src, slot});
+                    if (r.ready) {
+                        waits.push_back(r.ready);
+                        r.ready = nullptr;
+                    }
                     free_lane->reqs.push_back(s->pending.front());
                     s->pending.pop_front();
                 }
             }
             if (!entries.empty()) {
+                // inputs produced on a client stream: the lane waits for them
+                for (hipEvent_t e : waits) mdft::hip_check(hipStreamWaitEvent(free_lane->stream, e, 0), "wait input");
+                if (!waits.empty()) {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    s->free_events.insert(s->free_events.end(), waits.begin(), waits.end());
+                }
                 launch_micro_batch(s, *free_lane, entries);
                 free_lane->busy = true;
                 {
@@ -314,14 +363,16 @@ int mdft_server_prepare(mdft_server s, const uint32_t* lengths, uint32_t count)
     MDFT_API_BEGIN
     if (!s) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
     if (!lengths && count) throw Error(MDFT_ERR_INVALID, "mdft_server_prepare: lengths is NULL");
-    Context& ctx = Context::get(s->device);
-    for (uint32_t i = 0; i < count; ++i) ctx.line_plan((int)lengths[i]);
+    std::vector<std::shared_ptr<const mdft::LinePlan>> held;
+    hold_plans(s, lengths, count, held);
+    std::lock_guard<std::mutex> lk(s->mu);
+    commit_plans(s, held);
     MDFT_API_END
 }
 
-int mdft_server_submit(mdft_server s, uint32_t width, uint32_t height, int inverse, const void* src, uint64_t* ticket)
+static void server_submit(mdft_server s, uint32_t width, uint32_t height, int inverse, const void* src,
+                          hipStream_t producer, bool ordered, uint64_t* ticket)
 {
-    MDFT_API_BEGIN
     if (!s) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
     if (!src || !ticket) throw Error(MDFT_ERR_INVALID, "mdft_server_submit: src and ticket must not be NULL");
     mdft::batch_threads_for(&width, &height, 1);  // size checks
@@ -332,16 +383,52 @@ int mdft_server_submit(mdft_server s, uint32_t width, uint32_t height, int inver
     r.src = static_cast<const float2*>(src);
     r.bytes = sizeof(float2) * (uint64_t)width * height;
     r.t_submit = now_us();
+    const uint32_t lens[2] = {width, height};
+    std::vector<std::shared_ptr<const mdft::LinePlan>> held;
+    const uint64_t add = hold_plans(s, lens, 2, held);
     std::lock_guard<std::mutex> lk(s->mu);
     if (!s->sched_error.empty()) throw Error(MDFT_ERR_DEVICE, "server stopped: " + s->sched_error);
-    if (((r.bytes + 255) & ~uint64_t(255)) > s->arena.size())
-        throw Error(MDFT_ERR_BUDGET, "request of " + std::to_string(r.bytes) + " bytes exceeds the HBM budget of " +
+    if (slot_bytes(r.bytes) + s->table_bytes + add > s->arena.size())
+        throw Error(MDFT_ERR_BUDGET, "request of " + std::to_string(r.bytes) + " bytes (with " +
+                                         std::to_string(s->table_bytes + add) +
+                                         " bytes of resident tables) exceeds the HBM budget of " +
                                          std::to_string(s->arena.size()) + " bytes");
+    commit_plans(s, held);
+    if (ordered) {
+        mdft::DeviceGuard g(s->device);
+        hipEvent_t e = nullptr;
+        if (!s->free_events.empty()) {
+            e = s->free_events.back();
+            s->free_events.pop_back();
+        } else {
+            mdft::hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        }
+        const hipError_t rc = hipEventRecord(e, producer);
+        if (rc != hipSuccess) {
+            s->free_events.push_back(e);
+            mdft::hip_check(rc, "hipEventRecord");
+        }
+        r.ready = e;
+    }
     const uint64_t id = s->next_id++;
     s->reqs.emplace(id, r);
     s->pending.push_back(id);
     *ticket = id;
     s->cv_work.notify_one();
+}
+
+int mdft_server_submit(mdft_server s, uint32_t width, uint32_t height, int inverse, const void* src, uint64_t* ticket)
+{
+    MDFT_API_BEGIN
+    server_submit(s, width, height, inverse, src, nullptr, false, ticket);
+    MDFT_API_END
+}
+
+int mdft_server_submit_stream(mdft_server s, uint32_t width, uint32_t height, int inverse, const void* src,
+                              void* stream, uint64_t* ticket)
+{
+    MDFT_API_BEGIN
+    server_submit(s, width, height, inverse, src, static_cast<hipStream_t>(stream), true, ticket);
     MDFT_API_END
 }
 
@@ -408,6 +495,7 @@ int mdft_server_stats(mdft_server s, mdft_server_counters* out)
     out->launched_requests = s->launched_reqs;
     out->budget_bytes = s->arena.size();
     out->used_bytes = s->arena.used();
+    out->table_bytes = s->table_bytes;
     MDFT_API_END
 }
 
@@ -480,6 +568,9 @@ int mdft_server_destroy(mdft_server s)
             if (L.table_dev) (void)hipFree(L.table_dev);
             if (L.table_host) (void)hipHostFree(L.table_host);
         }
+        for (auto& kv : s->reqs)
+            if (kv.second.ready) (void)hipEventDestroy(kv.second.ready);
+        for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
         if (s->arena_mem) (void)hipFree(s->arena_mem);
     }
     delete s;

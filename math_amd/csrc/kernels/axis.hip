@@ -22,7 +22,7 @@ __host__ __device__ constexpr int axis_threads(int logm, bool wide)
 }
 
 template <int LOGM, int T, bool BLUE, bool COLS>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_axis(AxisJob job)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_axis(AxisJob job)
 {
     constexpr int LF = LineCfg<LOGM, T>::template lds_floats<COLS>();
     __shared__ float lds[LF > 0 ? LF : 1];
@@ -34,8 +34,11 @@ template <int LOGM, int T>
 static hipError_t launch_axis_tt(const AxisJob& job, bool blue, bool cols, hipStream_t stream)
 {
     constexpr int LPW = LineCfg<LOGM, T>::LPW;
-    const int grid = (job.count + LPW - 1) / LPW;
-    if (grid <= 0) return hipSuccess;
+    // in 64 bits: count may be up to 2^31 - 1
+    const int64_t grid64 = (int64_t(job.count) + LPW - 1) / LPW;
+    if (job.count < 0 || grid64 > 0x7fffffff) return hipErrorInvalidValue;
+    if (grid64 == 0) return hipSuccess;
+    const unsigned grid = (unsigned)grid64;
     if (blue) {
         if constexpr (LOGM >= 2) {
             if (cols)

@@ -29,7 +29,13 @@
 
 namespace mdft {
 
-__host__ __device__ constexpr int batch_maxlogm(int threads) { return threads >= 512 ? 13 : 12; }
+// longest padded length a tile of `threads` threads holds (one line per tile at most 8192 points)
+__host__ __device__ constexpr int batch_maxlogm(int threads)
+{
+    return (threads >= 1024 ? 10 : threads >= 512 ? 9 : 8) + fft_bits(13) > 13
+               ? 13
+               : (threads >= 1024 ? 10 : threads >= 512 ? 9 : 8) + fft_bits(13);
+}
 
 template <int LOGM, int T>
 constexpr int lds_need()
@@ -142,7 +148,7 @@ __device__ __forceinline__ void run_item(const BatchJob* __restrict__ jobs, uint
 // Four waves per SIMD (<= 128 VGPRs): 256-thread tiles (4 workgroups per
 // CU), 512-thread tiles (two) or 1024-thread tiles (one).
 template <int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_grid(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_batch_grid(
     const BatchJob* __restrict__ jobs, const uint2* __restrict__ items)
 {
     __shared__ float lds[batch_lds<T>()];
@@ -159,7 +165,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_b
 // Deadlock freedom of the dependency mode: an item waits only on items with
 // smaller tickets, taken by running workgroups that never wait on later ones.
 template <int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_persistent(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_batch_persistent(
     const BatchJob* __restrict__ jobs, const uint2* __restrict__ items, uint32_t nitems, uint32_t* ctl,
     uint32_t spin_limit)
 {
@@ -233,6 +239,9 @@ hipError_t launch_batch_persistent(int threads, const BatchJob* jobs, const uint
     if (threads == 256)
         hipLaunchKernelGGL(k_batch_persistent<256>, dim3(grid), dim3(256), 0, stream, jobs, items, nitems, ctl,
                            spin_limit);
+    else if (threads == 512)
+        hipLaunchKernelGGL(k_batch_persistent<512>, dim3(grid), dim3(512), 0, stream, jobs, items, nitems, ctl,
+                           spin_limit);
     else
         hipLaunchKernelGGL(k_batch_persistent<1024>, dim3(grid), dim3(1024), 0, stream, jobs, items, nitems, ctl,
                            spin_limit);
@@ -250,6 +259,8 @@ hipError_t batch_occupancy(int threads, int* blocks_per_cu)
 {
     if (threads == 256)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_persistent<256>, 256, 0);
+    if (threads == 512)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_persistent<512>, 512, 0);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch_persistent<1024>, 1024, 0);
 }
 

@@ -62,18 +62,38 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 // ---------------------------------------------------------------------------
 // compile-time line geometry
 // ---------------------------------------------------------------------------
-__host__ __device__ constexpr int ept_of(int logm) { return logm < 4 ? (1 << logm) : 16; }
-__host__ __device__ constexpr int npass_of(int logm) { return (logm + 3) / 4; }
-__host__ __device__ constexpr int pass_bits(int logm, int p) { return (logm - 4 * p) >= 4 ? 4 : (logm - 4 * p); }
+// Bits per radix pass of a line of padded length 2^logm: radix-16 passes (16
+// points per thread) or radix-8 passes (8 points per thread, half the
+// registers and LDS per thread).  A function of the length alone, so a line
+// runs the same schedule in every kernel.  16384-point lines keep radix 16 so
+// that one line fits a 1024-thread workgroup.
+#ifndef MDFT_FFT_BITS
+#define MDFT_FFT_BITS 4
+#endif
+static_assert(MDFT_FFT_BITS == 3 || MDFT_FFT_BITS == 4, "radix-8 or radix-16 passes");
+__host__ __device__ constexpr int fft_bits(int logm) { return logm >= 14 ? 4 : MDFT_FFT_BITS; }
+// minimum waves per SIMD the tile kernels are compiled for (register cap 512 / waves)
+#ifndef MDFT_TILE_WAVES
+#define MDFT_TILE_WAVES 4
+#endif
+__host__ __device__ constexpr int ept_of(int logm)
+{
+    return logm < fft_bits(logm) ? (1 << logm) : (1 << fft_bits(logm));
+}
+__host__ __device__ constexpr int npass_of(int logm) { return (logm + fft_bits(logm) - 1) / fft_bits(logm); }
+__host__ __device__ constexpr int pass_bits(int logm, int p)
+{
+    return (logm - fft_bits(logm) * p) >= fft_bits(logm) ? fft_bits(logm) : (logm - fft_bits(logm) * p);
+}
 
-// Twiddle rows of pass p >= 1 (Ns = 16^p before the pass, radix R): row k
+// Twiddle rows of pass p >= 1 (Ns = (2^fft_bits)^p before the pass, radix R): row k
 // (k < Ns) holds TW_ROW = 4 entries w^1, w^2, w^3, w^4 of
 // w = exp(-2*pi*i*k/(Ns*R)) (32 bytes, four rows per cache line); the other
 // powers are products of these (expand_twiddles).
 constexpr int TW_ROW = 4;
 __host__ __device__ constexpr int64_t tw_pass_len(int logm, int p)
 {
-    return (pass_bits(logm, p) > 0 ? (int64_t(1) << (4 * p)) : 0) * TW_ROW;
+    return (pass_bits(logm, p) > 0 ? (int64_t(1) << (fft_bits(logm) * p)) : 0) * TW_ROW;
 }
 __host__ __device__ constexpr int64_t tw_pass_off(int logm, int p)
 {
@@ -98,18 +118,21 @@ __host__ __device__ constexpr int64_t tw_off(int logm)
 __host__ __device__ constexpr int64_t tw_total() { return tw_off(MAX_LOGM + 1); }
 
 // LDS word of line position p in the exchange written by a pass of stride NS.
-// A pass writes its 16 outputs NS positions apart and the next pass reads 16
-// positions TPL apart.  When a 32-lane half-wave covers 32 consecutive
-// positions of one line (ONE_LINE), the reads are conflict-free unpadded and
-// the writes need: NS = 1 (lanes 16 apart) one pad word per 32, NS = 16
-// (lanes in two runs of 16, 256 apart) 16 pad words per 256, NS >= 256
-// (consecutive lanes) none.  Otherwise lanes span several lines and one pad
-// word per 16 with an odd line stride keeps them apart.  Every map satisfies
-// f(t + i*TPL) = f(t) + f(i*TPL) for t < TPL, so LDS offsets are immediates.
-template <int NS, bool ONE_LINE>
+// A pass writes its outputs NS positions apart and the next pass reads
+// positions TPL apart.  Radix-16 passes: when a 32-lane half-wave covers 32
+// consecutive positions of one line (ONE_LINE), the reads are conflict-free
+// unpadded and the writes need: NS = 1 (lanes 16 apart) one pad word per 32,
+// NS = 16 (lanes in two runs of 16, 256 apart) 16 pad words per 256, NS >= 256
+// (consecutive lanes) none; otherwise lanes span several lines and one pad
+// word per 16 with an odd line stride keeps them apart.  Radix-8 passes: one
+// pad word per 8.  Every map satisfies f(t + i*TPL) = f(t) + f(i*TPL) for
+// t < TPL, so LDS offsets are immediates.
+template <int NS, bool ONE_LINE, int B>
 __host__ __device__ constexpr int xpad(int p)
 {
-    if constexpr (!ONE_LINE) {
+    if constexpr (B == 3) {
+        return p + (p >> 3);
+    } else if constexpr (!ONE_LINE) {
         return p + (p >> 4);
     } else if constexpr (NS == 1) {
         return p + (p >> 5);
@@ -129,7 +152,8 @@ struct LineCfg {
     static constexpr int NPASS = npass_of(LOGM);
     static_assert(TPL <= THREADS, "line does not fit the workgroup");
     static_assert(LPW * TPL == THREADS, "workgroup must hold whole lines");
-    static constexpr int PADM = M + (M >> 4);
+    static constexpr int B = fft_bits(LOGM);
+    static constexpr int PADM = M + (M >> B);
     // a 32-lane half-wave covers 32 consecutive positions of one line
     template <bool COLS>
     static constexpr bool one_line()
@@ -374,7 +398,7 @@ struct TwLoad {
     static constexpr int R = 1 << pass_bits(LOGM, P);
     static constexpr int Q = tw_quads<R>();
     static constexpr int NB = C::EPT / R;
-    static constexpr int NS = 1 << (4 * P);
+    static constexpr int NS = 1 << (C::B * P);
     static_assert(NB * Q <= C::EPT / 2, "twiddle registers");
     __device__ __forceinline__ static void run(float4 (&w)[C::EPT / 2], int t, const float2* tw)
     {
@@ -427,7 +451,8 @@ struct FftPass {
     static constexpr int RB = pass_bits(LOGM, P);
     static constexpr int R = 1 << RB;
     static constexpr int NB = C::EPT / R;
-    static constexpr int NS = 1 << (4 * P);
+    static constexpr int NS = 1 << (C::B * P);
+    static constexpr int RADIX = 1 << C::B;
     static constexpr bool LAST = (P == C::NPASS - 1);
     static constexpr int LST = C::template lstride<COLS>();
     static constexpr bool ONE = C::template one_line<COLS>();
@@ -444,10 +469,10 @@ struct FftPass {
             exchange_sync<WAVE>();
             // layout of the previous pass (stride NS/16); the offset of slot i
             // is a compile-time constant (an LDS immediate)
-            const int q0 = xpad<NS / 16, ONE>(t);
+            const int q0 = xpad<NS / RADIX, ONE, C::B>(t);
 #pragma unroll
             for (int i = 0; i < C::EPT; ++i) {
-                const int q = q0 + xpad<NS / 16, ONE>(i * C::TPL);
+                const int q = q0 + xpad<NS / RADIX, ONE, C::B>(i * C::TPL);
                 v[i] = make_float2(re[q], im[q]);
             }
             if constexpr (LAST && PFL > PF) {
@@ -491,13 +516,13 @@ struct FftPass {
                 const int j = t + b * C::TPL;
                 const int k = j & (NS - 1);
                 const int base = (j - k) * R + k;
-                // NS == 1 (first pass, R == 16): base is a multiple of 16, so
-                // the 16 outputs are contiguous after padding; NS >= 16: the
-                // outputs lie in one padding block apart from whole blocks, so
-                // each step adds xpad(NS)
-                static_assert(NS == 1 ? R == 16 : NS % 16 == 0, "linear write offsets");
-                constexpr int step = xpad<NS, ONE>(NS);
-                const int q0 = xpad<NS, ONE>(base);
+                // NS == 1 (first pass, full radix): base is a multiple of the
+                // radix, so the outputs are contiguous after padding; NS >= the
+                // radix: the outputs lie in one padding block apart from whole
+                // blocks, so each step adds xpad(NS)
+                static_assert(NS == 1 ? R == RADIX : NS % RADIX == 0, "linear write offsets");
+                constexpr int step = xpad<NS, ONE, C::B>(NS);
+                const int q0 = xpad<NS, ONE, C::B>(base);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int q = q0 + r * step;

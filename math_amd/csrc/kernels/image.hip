@@ -21,7 +21,7 @@ __host__ __device__ constexpr int image_threads(int logm, bool wide)
 }
 
 template <int LOGM, int T, bool BLUE, bool COLS, int IO>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_image_axis(AxisJob job, ImageIO io)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_image_axis(AxisJob job, ImageIO io)
 {
     constexpr int LF = LineCfg<LOGM, T>::template lds_floats<COLS>();
     __shared__ float lds[LF > 0 ? LF : 1];
@@ -33,8 +33,10 @@ template <int LOGM, int T, bool COLS, int IO>
 static hipError_t launch_image_tt(const AxisJob& job, const ImageIO& io, bool blue, hipStream_t stream)
 {
     constexpr int LPW = LineCfg<LOGM, T>::LPW;
-    const int grid = (job.count + LPW - 1) / LPW;
-    if (grid <= 0) return hipSuccess;
+    const int64_t grid64 = (int64_t(job.count) + LPW - 1) / LPW;
+    if (job.count < 0 || grid64 > 0x7fffffff) return hipErrorInvalidValue;
+    if (grid64 == 0) return hipSuccess;
+    const unsigned grid = (unsigned)grid64;
     if (blue) {
         if constexpr (LOGM >= 2)
             hipLaunchKernelGGL((k_image_axis<LOGM, T, true, COLS, IO>), dim3(grid), dim3(T), 0, stream, job, io);

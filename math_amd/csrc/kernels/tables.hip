@@ -135,7 +135,7 @@ hipError_t gen_twiddles(float2* table, hipStream_t stream)
 {
     for (int logm = 1; logm <= MAX_LOGM; ++logm) {
         for (int p = 1; p < npass_of(logm); ++p) {
-            const int ns = 1 << (4 * p);
+            const int ns = 1 << (fft_bits(logm) * p);
             const int radix = 1 << pass_bits(logm, p);
             const int64_t len = int64_t(ns) * TW_ROW;
             float2* out = table + tw_off(logm) + tw_pass_off(logm, p);

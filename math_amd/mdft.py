@@ -57,7 +57,7 @@ class mdft_desc(ctypes.Structure):
 
 class mdft_server_counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("completed", "pending", "in_flight", "launches", "launched_requests",
-                                                "budget_bytes", "used_bytes")]
+                                                "budget_bytes", "used_bytes", "table_bytes")]
 
 
 class mdft_request(ctypes.Structure):
@@ -103,6 +103,9 @@ def lib():
                                 ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "mdft_exec": ([_vp, ctypes.c_int, _vp, ctypes.c_size_t], ctypes.c_int),
             "mdft_exec_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+            "mdft_cache_info": ([ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32),
+                                 ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+            "mdft_cache_set_limit": ([ctypes.c_uint64], ctypes.c_int),
             "mdft_tune_get": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
             "mdft_tune_set": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
             "mdft_tune_auto": ([ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
@@ -121,12 +124,18 @@ def lib():
                                   ctypes.c_int),
             "mdft_batch_exec_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
             "mdft_batch_check": ([_vp], ctypes.c_int),
+            "mdft_batch_exec_phase": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+            "mdft_batch_info": ([_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                 ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
             "mdft_batch_destroy": ([_vp], ctypes.c_int),
             "mdft_server_create": ([ctypes.POINTER(_vp), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32],
                                    ctypes.c_int),
             "mdft_server_prepare": ([_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32], ctypes.c_int),
             "mdft_server_submit": ([_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp,
                                     ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+            "mdft_server_submit_stream": ([_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp, _vp,
+                                           ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "mdft_server_wait": ([_vp, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_vp),
                                   ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
             "mdft_server_release": ([_vp, ctypes.c_uint64], ctypes.c_int),
@@ -152,6 +161,17 @@ def _check(rc):
 
 def init(device=0):
     _check(lib().mdft_init(device))
+
+
+def cache_info():
+    """(bytes, lengths, limit) of this thread's device table cache (mdft_cache_info)."""
+    b, n, lim = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint64()
+    _check(lib().mdft_cache_info(ctypes.byref(b), ctypes.byref(n), ctypes.byref(lim)))
+    return b.value, n.value, lim.value
+
+
+def cache_set_limit(nbytes):
+    _check(lib().mdft_cache_set_limit(int(nbytes)))
 
 
 def tune_get(logm, blue, cols):
@@ -340,6 +360,18 @@ class Batch:
     def exec_device(self, src_ptr, dst_ptr, stream=0):
         _check(lib().mdft_batch_exec_device(self._h, src_ptr, dst_ptr, stream))
 
+    def exec_phase(self, phase, src_ptr, dst_ptr, stream=0):
+        """Enqueue launch phase 1 (whole images, row tiles) or 2 (column tiles)."""
+        _check(lib().mdft_batch_exec_phase(self._h, int(phase), src_ptr, dst_ptr, stream))
+
+    def info(self):
+        """dict: items, algorithmic bytes and tile threads of each phase (mdft_batch_info)."""
+        i1, i2, b1, b2, t1, t2 = (ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64(),
+                                  ctypes.c_uint32(), ctypes.c_uint32())
+        _check(lib().mdft_batch_info(self._h, ctypes.byref(i1), ctypes.byref(i2), ctypes.byref(b1), ctypes.byref(b2),
+                                     ctypes.byref(t1), ctypes.byref(t2)))
+        return {"items": (i1.value, i2.value), "bytes": (b1.value, b2.value), "threads": (t1.value, t2.value)}
+
     def check(self):
         _check(lib().mdft_batch_check(self._h))
 
@@ -370,6 +402,13 @@ class Server:
         t = ctypes.c_uint64()
         _check(lib().mdft_server_submit(self._h, int(width), int(height), int(bool(inverse)), src_ptr,
                                         ctypes.byref(t)))
+        return t.value
+
+    def submit_stream(self, width, height, inverse, src_ptr, stream):
+        """Submit an input still being produced on `stream` (mdft_server_submit_stream)."""
+        t = ctypes.c_uint64()
+        _check(lib().mdft_server_submit_stream(self._h, int(width), int(height), int(bool(inverse)), src_ptr,
+                                               stream, ctypes.byref(t)))
         return t.value
 
     def wait(self, ticket, timeout_us=-1):

@@ -134,13 +134,26 @@ def test_config_640x480_fwd_inv(mdft):
     assert rt["nl2"] <= tol_nl2(640, 480), rt
 
 
+def check_against_ref32(got, x, inverse, label):
+    """Agreement with the FP32 restatement of the reference GPU pipeline
+    (oracle orc_dft2d_ref32, src/gpu/dft/dft.cpp:326-347) within
+    2e-6*log2(padded N)."""
+    h, w = x.shape
+    ref32 = O.dft2d_ref32(x, inverse, nthreads=8)
+    m = O.error_metrics(got.reshape(h, w), ref32.astype(np.complex128))
+    assert m["nl2"] <= tol_nl2(w, h), f"{label} {w}x{h} inv={inverse}: nl2 vs FP32 pipeline {m['nl2']:.3e}"
+    return m
+
+
 def test_config_1024x1024_round_trip(mdft):
     rng = np.random.default_rng(0x6D696C6C + 1)
     x = rand_c64(rng, (1024, 1024), "normal")
     log("1024x1024")
     y = run_exec(mdft, x, False)
     check_against_fp64(y, x, False, "C2 fwd")
+    check_against_ref32(y, x, False, "C2 fwd")
     z = run_exec(mdft, y, True)
+    check_against_ref32(z, y, True, "C2 inv")
     rt = O.error_metrics(z, x.astype(np.complex128))
     assert rt["nl2"] <= tol_nl2(1024, 1024), rt
     # Parseval: ||X||^2 = w*h*||x||^2
@@ -154,7 +167,9 @@ def test_config_4093x2047(mdft):
     x = rand_c64(rng, (2047, 4093), "uniform")
     log("4093x2047")
     for inv in (False, True):
-        check_against_fp64(run_exec(mdft, x, inv), x, inv, "C3")
+        got = run_exec(mdft, x, inv)
+        check_against_fp64(got, x, inv, "C3")
+        check_against_ref32(got, x, inv, "C3")
 
 
 def test_reference_large_6007x997(mdft):
@@ -163,7 +178,9 @@ def test_reference_large_6007x997(mdft):
     x = rand_c64(rng, (997, 6007), "uniform")
     log("6007x997")
     for inv in (False, True):
-        check_against_fp64(run_exec(mdft, x, inv), x, inv, "6007x997")
+        got = run_exec(mdft, x, inv)
+        check_against_fp64(got, x, inv, "6007x997")
+        check_against_ref32(got, x, inv, "6007x997")
 
 
 def test_replanning(mdft):

@@ -112,6 +112,23 @@ int mdft_exec(mdft_plan plan, int inverse, void* data, size_t count);
  * not ordered this way: do not replay two such graphs of one plan at once). */
 int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, void* stream);
 
+/* Allocation-free variant of mdft_exec_device for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc); no counterpart in the reference.  The
+ * same kernels transform a width x height device image without a plan
+ * object: the chirp, diagonal and twiddle tables of both sides must already
+ * be resident (keep a plan, batch or server of those lengths alive), and
+ * lines whose padded length exceeds 16384 use a static per-device scratch of
+ * 32 MiB (4 Mi complex<float>) instead of plan-owned memory.  Calls no
+ * hipMalloc, hipFree or other allocating routine.  Errors: MDFT_ERR_SIZE when
+ * a pass needs more scratch than that (message names the limit) or a side is
+ * outside [1, 2^29]; MDFT_ERR_STATE when the device has no context yet
+ * (mdft_init) or a length's tables are not resident.  Every call on a device
+ * shares its scratch: calls that overlap on one GPU (other streams, replayed
+ * graphs) race on it. */
+int mdft_exec_device_noalloc(uint32_t width, uint32_t height, int inverse, const void* src, void* dst,
+                             void* stream);
+
 /* ---- tile-shape tuning ---------------------------------------------------- */
 
 /* Tile shape the single-image kernels use for lines of padded length 2^logm

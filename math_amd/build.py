@@ -39,6 +39,7 @@ SOURCES = [
     "host/batch.cpp",
     "host/batch_builder.cpp",
     "host/server.cpp",
+    "host/noalloc.cpp",
 ]
 
 

@@ -131,6 +131,7 @@ Context::Context(int device) : device_(device)
     // FP64 table scratch for every tile-resident Bluestein length (2n + m doubles)
     scratch64_doubles_ = 2 * (size_t(LineMaxTile) / 2 + 1) + size_t(LineMaxTile);
     hip_check(hipMalloc(&scratch64_, sizeof(double) * scratch64_doubles_), "hipMalloc scratch");
+    noalloc_setup(device);
     // default tile shapes: wide tiles for columns of 1024 points and more
     for (int l = 0; l <= MAX_LOGM; ++l)
         for (int b = 0; b < 2; ++b) variant_[l][b][1] = (l >= 10) ? 1 : 0;

@@ -137,6 +137,13 @@ private:
     uint8_t variant_[MAX_LOGM + 1][2][2] = {};
 };
 
+// Static scratch of the allocation-free entry point (noalloc.cpp): its device
+// address is looked up once per device when the device's context is created
+// (the device is current then), and kept per device.
+void noalloc_setup(int device);
+float2* noalloc_scratch(int device);
+uint64_t noalloc_scratch_elems();
+
 
 // Orders the calls that share one scratch buffer (long-line plans) when a
 // caller enqueues them on different streams: a call on another stream than

@@ -103,6 +103,8 @@ def lib():
                                 ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "mdft_exec": ([_vp, ctypes.c_int, _vp, ctypes.c_size_t], ctypes.c_int),
             "mdft_exec_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+            "mdft_exec_device_noalloc": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp, _vp, _vp],
+                                         ctypes.c_int),
             "mdft_cache_info": ([ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32),
                                  ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "mdft_cache_set_limit": ([ctypes.c_uint64], ctypes.c_int),
@@ -161,6 +163,12 @@ def _check(rc):
 
 def init(device=0):
     _check(lib().mdft_init(device))
+
+
+def exec_device_noalloc(width, height, inverse, src_ptr, dst_ptr, stream=0):
+    """Allocation-free transform of a device image (mdft_exec_device_noalloc);
+    the tables of both sides must be resident (a live plan of the size)."""
+    _check(lib().mdft_exec_device_noalloc(int(width), int(height), int(bool(inverse)), src_ptr, dst_ptr, stream))
 
 
 def cache_info():

@@ -1,0 +1,88 @@
+"""GPU test of the allocation-free entry point mdft_exec_device_noalloc: it
+must give bitwise the same output as mdft_exec_device with a plan of the same
+size (the same kernels run; only the scratch of lines longer than one tile
+moves to a static device array), on two calls in a row whose second input is
+smaller, and refuse sizes beyond its static scratch and lengths whose tables
+are not resident."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.helpers import check_against_fp64, rand_c64
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    from math_amd import mdft as M
+
+    M.init(0)
+    return M, torch
+
+
+def plan_result(M, torch, plan, inverse, src):
+    out = torch.empty_like(src)
+    plan.exec_device(inverse, src.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    return out
+
+
+def test_noalloc_matches_exec_device_two_calls(env):
+    M, torch = env
+    rng = np.random.default_rng(4242)
+    # long rows (padded 65536, Bluestein: 3 x 65536 x 2 scratch elements), then
+    # a smaller long-column case (padded 32768, power of two), then a tile-resident one
+    cases = [(20000, 3, False), (2, 16385, True), (640, 480, True)]
+    plans = {(w, h): M.Plan(w, h) for (w, h, _) in cases}  # keep the tables resident
+    s = torch.cuda.current_stream().cuda_stream
+    for (w, h, inv) in cases:
+        x = rand_c64(rng, (h, w))
+        src = torch.from_numpy(x.ravel().copy()).cuda()
+        dst = torch.full_like(src, float("nan"))
+        M.exec_device_noalloc(w, h, inv, src.data_ptr(), dst.data_ptr(), s)
+        ref = plan_result(M, torch, plans[(w, h)], inv, src)
+        torch.cuda.synchronize()
+        assert torch.equal(dst, ref), f"{w}x{h} inv={inv}: noalloc output differs from mdft_exec_device"
+        assert np.array_equal(src.cpu().numpy(), x.ravel()), "input modified"
+        check_against_fp64(dst.cpu().numpy(), x, inv, "noalloc")
+    for p in plans.values():
+        p.close()
+
+
+def test_noalloc_in_place(env):
+    M, torch = env
+    rng = np.random.default_rng(7)
+    w, h = 16385, 2
+    p = M.Plan(w, h)
+    x = rand_c64(rng, (h, w))
+    a = torch.from_numpy(x.ravel().copy()).cuda()
+    ref = plan_result(M, torch, p, False, a)
+    M.exec_device_noalloc(w, h, False, a.data_ptr(), a.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert torch.equal(a, ref)
+    p.close()
+
+
+def test_noalloc_errors(env):
+    M, torch = env
+    buf = torch.zeros(16, dtype=torch.complex64, device="cuda")
+    # 40000 x 40: the row pass needs 40 x 65536 x 2 scratch elements, above 4 Mi
+    with pytest.raises(M.ErrorException, match="above the static limit") as e:
+        M.exec_device_noalloc(40000, 40, False, buf.data_ptr(), buf.data_ptr())
+    assert e.value.code == M.MDFT_ERR_SIZE
+    with pytest.raises(M.ErrorException, match="for compute m"):
+        M.exec_device_noalloc(0, 5, False, buf.data_ptr(), buf.data_ptr())
+    # a length no plan holds: free every unused table first
+    _, _, lim = M.cache_info()
+    M.cache_set_limit(0)
+    try:
+        with pytest.raises(M.ErrorException, match="not resident") as e:
+            M.exec_device_noalloc(4001, 3, False, buf.data_ptr(), buf.data_ptr())
+        assert e.value.code == M.MDFT_ERR_STATE
+    finally:
+        M.cache_set_limit(lim)
+    rc = M.lib().mdft_exec_device_noalloc(4, 4, 0, None, None, None)
+    assert rc == M.MDFT_ERR_DATA_SIZE

@@ -334,8 +334,9 @@ def single_transforms(M, torch, reps=20):
     """One image per call on device memory (mdft_exec_device), event-timed: the
     single-image configurations 640x480 (forward and inverse), 1024x1024 (round
     trip) and 4093x2047, with their algorithmic bandwidth 32*w*h bytes per
-    transform; also the image path on a 640x480 rectangle and 4096 FFTs of
-    4096 points (16 bytes per point)."""
+    transform; lines longer than one tile (20000x3, 3x32768, 65536x1,
+    16385x64: the four-step path); also the image path on a 640x480 rectangle
+    and 4096 FFTs of 4096 points (16 bytes per point)."""
     res = {}
     sh = torch.cuda.current_stream().cuda_stream
     g = torch.Generator(device="cuda:0")
@@ -379,6 +380,19 @@ def single_transforms(M, torch, reps=20):
     run_img = lambda: img.exec_device(pix.data_ptr(), 700, 500, 700, 30, 10, mag.data_ptr(), 640, sh)
     res["image_640x480_rgba32f"] = {"us": round(_event_us(torch, run_img, reps), 1)}
     img.close()
+    # lines longer than one tile (padded length above 16384): the four-step path
+    for (w, h) in [(20000, 3), (3, 32768), (65536, 1), (16385, 64)]:
+        x = torch.randn(w * h, dtype=torch.complex64, device="cuda:0", generator=g)
+        y = torch.empty_like(x)
+        p = M.Plan(w, h)
+        us = _event_us(torch, lambda: p.exec_device(False, x.data_ptr(), y.data_ptr(), sh), reps)
+        xin = x.cpu().numpy().reshape(h, w)
+        m = _errors(y.cpu().numpy().reshape(h, w), _fft2_64(xin, False))
+        res[f"{w}x{h}_fwd"] = {"us": round(us, 1), "alg_gbs": round(32.0 * w * h / (us * 1e-6) / 1e9, 1),
+                               "nl2": float("%.3e" % m["nl2"]),
+                               "bound": float("%.3e" % (2e-6 * max(1.0, math.log2(_padded(w) * _padded(h)))))}
+        p.close()
+        del x, y
     f = M.Fft(4096, 4096)
     z = torch.randn(4096 * 4096, dtype=torch.complex64, device="cuda:0", generator=g)
     us = _event_us(torch, lambda: f.exec_device(False, z.data_ptr(), z.data_ptr(), sh), reps)

@@ -131,14 +131,17 @@ int mdft_exec_device_noalloc(uint32_t width, uint32_t height, int inverse, const
 
 /* ---- tile-shape tuning ---------------------------------------------------- */
 
-/* Tile shape the single-image kernels use for lines of padded length 2^logm
- * (1 <= logm <= 14), Bluestein (blue != 0) or power-of-two, along rows
- * (cols == 0) or columns: 0 = 256-thread tiles, 1 = 1024-thread tiles (four
- * times the lines per tile).  No counterpart in the reference, whose tile
- * size follows the device's shared-memory limit (src/gpu/dft/fft.cpp:51-61,
- * function.cpp:94-118); this build picks it per shape by measurement. */
-int mdft_tune_get(int logm, int blue, int cols, int* variant);
-int mdft_tune_set(int logm, int blue, int cols, int variant);
+/* Tile shape the single-image kernels use for lines whose FFT has the given
+ * shape: 1..14 for 2^shape points, 38..43 for 3 * 2^(shape - 32) points (the
+ * Bluestein convolution length used when 3 * 2^k >= 2n - 1 is shorter than
+ * the reference's compute_m(n)); Bluestein (blue != 0) or power-of-two, along
+ * rows (cols == 0) or columns: 0 = 256-thread tiles, 1 = 1024-thread tiles
+ * (four times the lines per tile).  No counterpart in the reference, whose
+ * tile size follows the device's shared-memory limit
+ * (src/gpu/dft/fft.cpp:51-61, function.cpp:94-118); this build picks it per
+ * shape by measurement. */
+int mdft_tune_get(int shape, int blue, int cols, int* variant);
+int mdft_tune_set(int shape, int blue, int cols, int variant);
 
 /* Autotuner: time both tile shapes of every (padded length, Bluestein, axis)
  * kernel on this thread's device (64 MiB of lines each, in a private buffer)
@@ -147,7 +150,7 @@ int mdft_tune_set(int logm, int blue, int cols, int variant);
  * narrow one runs regardless of the table. */
 int mdft_tune_auto(double* seconds);
 
-/* Save / load the table as text, one "logm blue cols variant" line per entry
+/* Save / load the table as text, one "shape blue cols variant" line per entry
  * ('#' starts a comment).  A file with a bad entry is rejected as a whole
  * (MDFT_ERR_INVALID). */
 int mdft_tune_save(const char* path);
@@ -303,6 +306,12 @@ typedef struct {
 
 /* streams = 0 and max_batch = 0 select the defaults (4 and 128). */
 int mdft_server_create(mdft_server* server, uint64_t hbm_budget_bytes, uint32_t streams, uint32_t max_batch);
+
+/* Batching window (no counterpart in the reference): while another
+ * micro-batch is on the device, waiting requests launch together once
+ * min_batch of them wait or the oldest has waited max_wait_us; an idle device
+ * launches at once.  Defaults 1 and 0 (launch whenever a lane is free). */
+int mdft_server_set_batching(mdft_server server, uint32_t min_batch, double max_wait_us);
 
 /* Build the device-resident line plans of these lengths ahead of traffic. */
 int mdft_server_prepare(mdft_server server, const uint32_t* lengths, uint32_t count);

@@ -11,6 +11,7 @@
 #include "batch_builder.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <string>
 #include <tuple>
@@ -24,17 +25,20 @@ void arrange_for_xcds(const std::vector<uint2>& items, const std::vector<double>
 int batch_threads_for(const uint32_t* widths, const uint32_t* heights, size_t count)
 {
     const int limit = batch_max_logm(1024);
-    int maxlogm = 0;
+    int tpl = 1;
     for (size_t i = 0; i < count; ++i) {
         const int l1 = ilog2(compute_m(widths[i])), l2 = ilog2(compute_m(heights[i]));
         if (l1 > limit || l2 > limit)
             throw Error(MDFT_ERR_SIZE, "batch transform " + std::to_string(i) + " (" + std::to_string(widths[i]) + "x" +
                                            std::to_string(heights[i]) + ") exceeds the batch padded-length limit " +
                                            std::to_string(1 << limit));
-        maxlogm = std::max({maxlogm, l1, l2});
+        for (uint32_t n : {widths[i], heights[i]}) {
+            const int s = line_shape(n);
+            tpl = std::max(tpl, m_of(s) / ept_of(s));
+        }
     }
     // the smallest tile that holds a whole line of every length
-    return maxlogm <= batch_max_logm(256) ? 256 : maxlogm <= batch_max_logm(512) ? 512 : 1024;
+    return tpl <= 256 ? 256 : tpl <= 512 ? 512 : 1024;
 }
 
 void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, int TC, bool dependent,
@@ -56,7 +60,7 @@ void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, i
 
     struct Item {
         int phase;
-        int logm, blue, kind;  // kind: 0 whole, 1 row tile, 2 column tile
+        int m, shape, blue, kind;  // kind: 0 whole, 1 row tile, 2 column tile
         uint32_t seq;
         uint2 v;
     };
@@ -70,13 +74,13 @@ void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, i
         auto cp = ctx.line_plan(h);
         out.keep.push_back(rp);
         out.keep.push_back(cp);
-        const int lpw_r = w > 1 ? batch_lines_per_tile(T, rp->logm) : 0;
+        const int lpw_r = w > 1 ? batch_lines_per_tile(T, rp->shape) : 0;
         // a whole image runs both passes in one row-phase workgroup
-        const int lpw_cw = h > 1 ? batch_lines_per_tile(T, cp->logm) : 0;
+        const int lpw_cw = h > 1 ? batch_lines_per_tile(T, cp->shape) : 0;
         const bool whole = (w > 1 && h > 1 && h <= lpw_r && w <= lpw_cw);
         // column tiles of images with a row pass run in the column phase (TC
         // threads); a single-column-pass image (w == 1) runs with the rows
-        const int lpw_c = h > 1 ? batch_lines_per_tile((w > 1 && !whole) ? TC : T, cp->logm) : 0;
+        const int lpw_c = h > 1 ? batch_lines_per_tile((w > 1 && !whole) ? TC : T, cp->shape) : 0;
         const int ntr = w > 1 ? (h + lpw_r - 1) / lpw_r : 0;
         const int ntc = h > 1 ? (w + lpw_c - 1) / lpw_c : 0;
         const int sig = (dependent && w > 1 && h > 1 && !whole) ? (int)counter++ : -1;
@@ -84,7 +88,7 @@ void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, i
         if (w > 1) {
             BatchJob j{};
             j.job = ctx.make_job(*rp, e.src, e.dst, 1, w, h, e.inverse);
-            j.logm = rp->logm;
+            j.shape = rp->shape;
             j.blue = rp->blue;
             j.cols = 0;
             j.ntiles = ntr;
@@ -99,7 +103,7 @@ void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, i
             BatchJob j{};
             // the column pass reads the row pass's output when there is one
             j.job = ctx.make_job(*cp, w > 1 ? e.dst : e.src, e.dst, w, 1, w, e.inverse);
-            j.logm = cp->logm;
+            j.shape = cp->shape;
             j.blue = cp->blue;
             j.cols = 1;
             j.ntiles = ntc;
@@ -113,15 +117,15 @@ void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, i
         }
         std::vector<uint2> ri, ci;
         if (whole) {
-            items.push_back({1, rp->logm, rp->blue, 0, seq++, make_uint2(rj, BATCH_WHOLE | (uint32_t)cj)});
+            items.push_back({1, rp->m, rp->shape, rp->blue, 0, seq++, make_uint2(rj, BATCH_WHOLE | (uint32_t)cj)});
             ri.push_back(items.back().v);
         } else {
             for (int t = 0; t < ntr; ++t) {
-                items.push_back({1, rp->logm, rp->blue, 1, seq++, make_uint2(rj, t)});
+                items.push_back({1, rp->m, rp->shape, rp->blue, 1, seq++, make_uint2(rj, t)});
                 ri.push_back(items.back().v);
             }
             for (int t = 0; t < ntc; ++t) {
-                items.push_back({w > 1 ? 2 : 1, cp->logm, cp->blue, 2, seq++, make_uint2(cj, t)});
+                items.push_back({w > 1 ? 2 : 1, cp->m, cp->shape, cp->blue, 2, seq++, make_uint2(cj, t)});
                 ci.push_back(items.back().v);
             }
         }
@@ -141,15 +145,15 @@ void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, i
         out.nphase1 = (uint32_t)out.items.size();
     } else {
         std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) {
-            return std::make_tuple(a.phase, -a.logm, a.blue, a.kind, a.seq) <
-                   std::make_tuple(c.phase, -c.logm, c.blue, c.kind, c.seq);
+            return std::make_tuple(a.phase, -a.m, a.shape, a.blue, a.kind, a.seq) <
+                   std::make_tuple(c.phase, -c.m, c.shape, c.blue, c.kind, c.seq);
         });
         std::vector<uint2> p1, p2;
         std::vector<double> c1, c2;
         for (const Item& it : items) {
             // cost of a tile ~ points x passes (x2 for Bluestein's two FFTs)
-            const int lpw = batch_lines_per_tile(it.kind == 2 && it.phase == 2 ? TC : T, it.logm);
-            const double cost = double(lpw) * double(1 << it.logm) * (it.logm + 2) * (it.blue ? 2.0 : 1.0) *
+            const int lpw = batch_lines_per_tile(it.kind == 2 && it.phase == 2 ? TC : T, it.shape);
+            const double cost = double(lpw) * double(it.m) * (std::log2(double(it.m)) + 2) * (it.blue ? 2.0 : 1.0) *
                                 (it.kind == 0 ? 2.0 : 1.0);
             (it.phase == 1 ? p1 : p2).push_back(it.v);
             (it.phase == 1 ? c1 : c2).push_back(cost);

@@ -92,8 +92,8 @@ int mdft_plan_info(mdft_plan plan, uint32_t* m1, uint32_t* m2, uint64_t* device_
 {
     MDFT_API_BEGIN
     if (!plan) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
-    if (m1) *m1 = (uint32_t)plan->row->m;
-    if (m2) *m2 = (uint32_t)plan->col->m;
+    if (m1) *m1 = (uint32_t)plan->row->padded;
+    if (m2) *m2 = (uint32_t)plan->col->padded;
     if (device_bytes)
         *device_bytes = plan->row->bytes + (plan->col != plan->row ? plan->col->bytes : 0) +
                         (plan->xd ? sizeof(float2) * (uint64_t)plan->w * plan->h : 0) +
@@ -155,10 +155,13 @@ int mdft_cache_set_limit(uint64_t bytes)
     MDFT_API_END
 }
 
-static void check_tune_args(int logm, int blue, int cols)
+static void check_tune_args(int shape, int blue, int cols)
 {
-    if (logm < 1 || logm > mdft::MAX_LOGM || (blue != 0 && blue != 1) || (cols != 0 && cols != 1))
-        throw Error(MDFT_ERR_INVALID, "tune: logm must be in [1, 14], blue and cols 0 or 1");
+    const bool ok_shape = (shape >= 1 && shape <= mdft::MAX_LOGM) ||
+                          (mdft::is_f3(shape) && mdft::f3_k(shape) >= mdft::F3_MIN_K &&
+                           mdft::f3_k(shape) <= mdft::F3_MAX_K);
+    if (!ok_shape || (blue != 0 && blue != 1) || (cols != 0 && cols != 1))
+        throw Error(MDFT_ERR_INVALID, "tune: shape must be in [1, 14] or [38, 43], blue and cols 0 or 1");
 }
 
 int mdft_tune_get(int logm, int blue, int cols, int* variant)

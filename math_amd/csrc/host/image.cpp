@@ -115,7 +115,7 @@ int mdft_image_exec_device(mdft_image image, const void* src, uint32_t src_width
         io.mel = 1, io.mls = (int64_t)dst_pitch;
         const mdft::AxisJob j = ctx.make_job(row, nullptr, image->buf, 1, w, h, false);
         const int mode = mdft::IO_LUM | (h == 1 ? mdft::IO_MAG : 0);
-        launch(mdft::launch_image_axis(row.logm, j, io, row.blue, false, mode, false, s), "image row pass launch");
+        launch(mdft::launch_image_axis(row.shape, j, io, row.blue, false, mode, false, s), "image row pass launch");
     }
     if (h > 1) {
         // columns: magnitude out; luminance in as well when there was no row pass
@@ -123,8 +123,8 @@ int mdft_image_exec_device(mdft_image image, const void* src, uint32_t src_width
         io.mel = (int64_t)dst_pitch, io.mls = 1;
         const mdft::AxisJob j = ctx.make_job(col, image->buf, nullptr, w, 1, w, false);
         const int mode = mdft::IO_MAG | (w == 1 ? mdft::IO_LUM : 0);
-        const bool wide = ctx.wide_tiles(col.logm, col.blue, true, w);
-        launch(mdft::launch_image_axis(col.logm, j, io, col.blue, true, mode, wide, s), "image column pass launch");
+        const bool wide = ctx.wide_tiles(col.shape, col.blue, true, w);
+        launch(mdft::launch_image_axis(col.shape, j, io, col.blue, true, mode, wide, s), "image column pass launch");
     }
     image->order.after(s);
     MDFT_API_END

@@ -37,6 +37,16 @@ int compute_m(long long n)
     return (int)m;
 }
 
+int line_shape(long long n)
+{
+    const int m = compute_m(n);
+    if (is_pow2((uint64_t)n) || m > (1 << MAX_LOGM)) return ilog2((uint64_t)m);
+    long long m3 = 3LL << F3_MIN_K;
+    while (m3 < 2 * n - 1) m3 <<= 1;
+    if (m3 < m && m3 <= (3LL << F3_MAX_K)) return F3_BASE + ilog2((uint64_t)(m3 / 3));
+    return ilog2((uint64_t)m);
+}
+
 LinePlan::~LinePlan()
 {
     if (chirp) (void)hipFree(chirp);
@@ -133,8 +143,10 @@ Context::Context(int device) : device_(device)
     hip_check(hipMalloc(&scratch64_, sizeof(double) * scratch64_doubles_), "hipMalloc scratch");
     noalloc_setup(device);
     // default tile shapes: wide tiles for columns of 1024 points and more
-    for (int l = 0; l <= MAX_LOGM; ++l)
-        for (int b = 0; b < 2; ++b) variant_[l][b][1] = (l >= 10) ? 1 : 0;
+    for (int i = 0; i < NSHAPES; ++i) {
+        const int shape = i <= MAX_LOGM ? i : F3_BASE + F3_MIN_K + (i - MAX_LOGM - 1);
+        for (int b = 0; b < 2; ++b) variant_[i][b][1] = (m_of(shape) >= 1024) ? 1 : 0;
+    }
 }
 
 std::shared_ptr<const LinePlan> Context::line_plan(int n)
@@ -149,8 +161,10 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
 
     auto lp = std::make_shared<LinePlan>();
     lp->n = n;
-    lp->m = m;
-    lp->logm = ilog2((uint64_t)m);
+    lp->shape = line_shape(n);
+    lp->padded = m;
+    lp->m = is_f3(lp->shape) ? m_of(lp->shape) : m;
+    lp->logm = is_f3(lp->shape) ? 0 : ilog2((uint64_t)m);
     lp->blue = !is_pow2((uint64_t)n);
     if (lp->logm > LARGE_MAX_LOGM)
         throw Error(MDFT_ERR_SIZE, "Error size " + std::to_string(n) + ": padded length " + std::to_string(m) +
@@ -169,7 +183,7 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
     if (lp->blue) {
         // the FP64 scratch is sized at context creation for every tile-resident
         // length; only long lines grow it (hipFree waits for the device)
-        const size_t need = 2 * (size_t)n + (size_t)m;
+        const size_t need = 2 * (size_t)n + (size_t)lp->m;
         if (need > scratch64_doubles_) {
             hip_check(hipStreamSynchronize(gen_stream_), "table generation");
             if (scratch64_) hip_check(hipFree(scratch64_), "hipFree");
@@ -178,20 +192,21 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
             hip_check(hipMalloc(&scratch64_, sizeof(double) * need), "hipMalloc scratch");
             scratch64_doubles_ = need;
         }
+        const int cm = lp->m;  // convolution length
         hip_check(hipMalloc(&lp->chirp, sizeof(float2) * (size_t)n), "hipMalloc chirp");
-        hip_check(hipMalloc(&lp->dtab, sizeof(float2) * (size_t)m), "hipMalloc dtab");
-        lp->bytes += sizeof(float2) * ((size_t)n + (size_t)m);
+        hip_check(hipMalloc(&lp->dtab, sizeof(float2) * (size_t)cm), "hipMalloc dtab");
+        lp->bytes += sizeof(float2) * ((size_t)n + (size_t)cm);
         double2* chirp64 = reinterpret_cast<double2*>(scratch64_);
         if (!lp->large()) {
-            hip_check(hipMalloc(&lp->dir, sizeof(float2) * (size_t)dir_tables_len(n, m)), "hipMalloc tables");
-            lp->bytes += sizeof(float2) * (size_t)dir_tables_len(n, m);
+            hip_check(hipMalloc(&lp->dir, sizeof(float2) * (size_t)dir_tables_len(n, cm)), "hipMalloc tables");
+            lp->bytes += sizeof(float2) * (size_t)dir_tables_len(n, cm);
         }
-        hip_check(gen_chirp(chirp64, lp->chirp, lp->dir, n, m, gen_stream_), "chirp generation");
+        hip_check(gen_chirp(chirp64, lp->chirp, lp->dir, n, cm, gen_stream_), "chirp generation");
         if (!lp->large()) {
             // D = FFT_M(h2)/M by a direct FP64 sum with exact index reduction
-            hip_check(gen_dtab(lp->dtab, chirp64, scratch64_ + 2 * (size_t)n, n, m, gen_stream_),
+            hip_check(gen_dtab(lp->dtab, chirp64, scratch64_ + 2 * (size_t)n, n, cm, gen_stream_),
                       "diagonal generation");
-            hip_check(gen_dir_diagonal(lp->dir, lp->dtab, n, m, gen_stream_), "diagonal tables");
+            hip_check(gen_dir_diagonal(lp->dir, lp->dtab, n, cm, gen_stream_), "diagonal tables");
         } else {
             // long lines: D = FFT_M(h2)/M by this library's four-step FFT on the
             // FP32-rounded embedding (the reference computes its D with an FP32 FFT
@@ -292,7 +307,7 @@ AxisJob Context::make_job(const LinePlan& lp, const float2* src, float2* dst, in
     j.chirp = lp.dir ? lp.dir + (inverse ? n : 0) : nullptr;
     j.dtab = lp.dir ? lp.dir + 2 * n + (inverse ? m : 0) : nullptr;
     j.chirp_out = lp.dir ? lp.dir + 2 * n + 2 * m + (inverse ? n : 0) : nullptr;
-    j.tw = twiddles(lp.logm);
+    j.tw = twiddles(lp.shape);
     return j;
 }
 
@@ -380,27 +395,27 @@ void Context::enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, i
         return;
     }
     const AxisJob j = make_job(lp, src, dst, es, ls, count, inverse);
-    hip_check(launch_axis(lp.logm, j, lp.blue, cols, tile_threads(lp.logm, lp.blue, cols, count), stream),
+    hip_check(launch_axis(lp.shape, j, lp.blue, cols, tile_threads(lp.shape, lp.blue, cols, count), stream),
               "axis pass launch");
 }
 
-bool Context::wide_tiles(int logm, bool blue, bool cols, int count) const
+bool Context::wide_tiles(int shape, bool blue, bool cols, int count) const
 {
-    if (variant(logm, blue, cols) == 0) return false;
+    if (variant(shape, blue, cols) == 0) return false;
     // wide tiles hold 4x the lines of narrow ones; keep at least one tile per CU
-    const int tpl = (1 << logm) / ept_of(logm);
+    const int tpl = m_of(shape) / ept_of(shape);
     const int lines_wide = tpl >= 1024 ? 1 : 1024 / tpl;
     const int64_t tiles = (int64_t(count) + lines_wide - 1) / lines_wide;
     return tiles >= int64_t(cu_count_);
 }
 
-int Context::tile_threads(int logm, bool blue, bool cols, int count) const
+int Context::tile_threads(int shape, bool blue, bool cols, int count) const
 {
-    if (wide_tiles(logm, blue, cols, count)) return 1024;
+    if (wide_tiles(shape, blue, cols, count)) return 1024;
     // passes with few lines: smaller tiles (one wave for lines of up to 1024
     // points) so every CU gets work; rows aim at four tiles per CU, columns at
     // one, since narrower column tiles read shorter runs of each row
-    const int tpl = (1 << logm) / ept_of(logm);
+    const int tpl = m_of(shape) / ept_of(shape);
     const int64_t want = int64_t(cols ? 1 : 4) * cu_count_;
     int t = tpl > 256 ? tpl : 256;
     while (t > 64 && t / 2 >= tpl && (int64_t(count) + t / tpl - 1) / (t / tpl) < want) t /= 2;

@@ -33,12 +33,22 @@ void hip_check(hipError_t e, const char* what);
 int compute_m(long long n);
 int ilog2(uint64_t v);
 bool is_pow2(uint64_t v);
+// FFT shape (fft_device.hpp) of a line of length n: log2(n) for a power of
+// two; for other lengths the Bluestein convolution length, the reference's
+// compute_m(n) = 2^j or 3 * 2^k >= 2n - 1 when that is shorter (3 * 2^k in
+// [192, 6144]); lines longer than one tile keep compute_m.
+int line_shape(long long n);
+// index of a shape in per-shape tables, 0 .. NSHAPES - 1
+constexpr int NSHAPES = MAX_LOGM + 1 + (F3_MAX_K - F3_MIN_K + 1);
+inline int shape_index(int shape) { return is_f3(shape) ? MAX_LOGM + 1 + (f3_k(shape) - F3_MIN_K) : shape; }
 
 // Device tables of one line length n.
 struct LinePlan {
     int n = 0;
-    int m = 1;         // FFT length: n for a power of two, compute_m(n) otherwise
-    int logm = 0;
+    int m = 1;         // FFT length: n for a power of two, the Bluestein convolution length otherwise
+    int logm = 0;      // log2(m) for power-of-two m (lines longer than one tile: the four-step split)
+    int shape = 0;     // FFT shape of the tile kernels (line_shape(n))
+    int padded = 1;    // the reference's padded length compute_m(n) (reported by mdft_plan_info)
     bool blue = false;  // Bluestein (n not a power of two)
     float2* chirp = nullptr;  // n entries, Bluestein only
     float2* dtab = nullptr;   // m entries, Bluestein only
@@ -75,7 +85,7 @@ public:
 
     int device() const { return device_; }
     int cu_count() const { return cu_count_; }
-    const float2* twiddles(int logm) const { return twiddles_ + tw_off(logm); }
+    const float2* twiddles(int shape) const { return twiddles_ + tw_off(shape); }
 
     // Cached, device-resident plan of one line length (thread-safe).  The
     // cache holds at most cache_limit() bytes of tables that no caller holds:
@@ -109,13 +119,13 @@ public:
                      bool inverse) const;
 
     // Tile-shape table of the single-image kernels: variant 0 = narrow
-    // (256-thread) tiles, 1 = wide (1024-thread) tiles, per (logm, blue, cols).
-    int variant(int logm, bool blue, bool cols) const { return variant_[logm][blue][cols]; }
+    // (256-thread) tiles, 1 = wide (1024-thread) tiles, per (FFT shape, blue, cols).
+    int variant(int shape, bool blue, bool cols) const { return variant_[shape_index(shape)][blue][cols]; }
     // the table's choice, narrowed when wide tiles would leave CUs without work
-    bool wide_tiles(int logm, bool blue, bool cols, int count) const;
+    bool wide_tiles(int shape, bool blue, bool cols, int count) const;
     // workgroup size of a single-image pass of `count` lines (launch_axis)
-    int tile_threads(int logm, bool blue, bool cols, int count) const;
-    void set_variant(int logm, bool blue, bool cols, int v) { variant_[logm][blue][cols] = (uint8_t)v; }
+    int tile_threads(int shape, bool blue, bool cols, int count) const;
+    void set_variant(int shape, bool blue, bool cols, int v) { variant_[shape_index(shape)][blue][cols] = (uint8_t)v; }
 
 private:
     explicit Context(int device);
@@ -134,7 +144,7 @@ private:
     uint64_t use_clock_ = 0;
     uint64_t cache_limit_ = uint64_t(4) << 30;
     void trim_locked(uint64_t limit);
-    uint8_t variant_[MAX_LOGM + 1][2][2] = {};
+    uint8_t variant_[NSHAPES][2][2] = {};
 };
 
 // Static scratch of the allocation-free entry point (noalloc.cpp): its device

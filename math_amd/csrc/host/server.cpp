@@ -127,6 +127,10 @@ struct mdft_server_s {
     Arena arena;
     char* arena_mem = nullptr;
     uint32_t max_batch = 128;
+    // batching window: while another micro-batch runs, a new one launches once
+    // min_batch requests wait or the oldest has waited max_wait_us
+    uint32_t min_batch = 1;
+    double max_wait_us = 0.0;
     std::vector<Lane> lanes;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -265,14 +269,18 @@ void scheduler(mdft_server_s* s)
                 }
             std::vector<mdft::BatchEntry> entries;
             std::vector<hipEvent_t> waits;
+            bool holding = false;  // requests wait for the batching window
             if (free_lane) {
+                bool any_busy = false;
+                for (Lane& L : s->lanes) any_busy |= L.busy;
                 std::lock_guard<std::mutex> lk(s->mu);
-                if (s->stop && s->pending.empty()) {
-                    bool any_busy = false;
-                    for (Lane& L : s->lanes) any_busy |= L.busy;
-                    if (!any_busy) return;
-                }
-                while (!s->pending.empty() && entries.size() < s->max_batch) {
+                if (s->stop && s->pending.empty() && !any_busy) return;
+                // an idle device launches at once; a busy one lets requests
+                // accumulate into the next micro-batch for up to max_wait_us
+                if (any_busy && !s->stop && !s->pending.empty() && s->pending.size() < s->min_batch &&
+                    now_us() - s->reqs[s->pending.front()].t_submit < s->max_wait_us)
+                    holding = true;
+                while (!holding && !s->pending.empty() && entries.size() < s->max_batch) {
                     Request& r = s->reqs[s->pending.front()];
                     uint64_t off = 0;
                     // slots in use + this slot + resident tables within the budget
@@ -312,7 +320,7 @@ void scheduler(mdft_server_s* s)
                 for (Lane& L : s->lanes) any_busy |= L.busy;
                 std::unique_lock<std::mutex> lk(s->mu);
                 if (s->stop && s->pending.empty() && !any_busy) return;
-                if (any_busy) {
+                if (any_busy || holding) {
                     lk.unlock();
                     std::this_thread::yield();  // poll the in-flight events
                 } else {
@@ -355,6 +363,18 @@ int mdft_server_create(mdft_server* server, uint64_t hbm_budget_bytes, uint32_t 
     }
     s->sched = std::thread(scheduler, s.get());
     *server = s.release();
+    MDFT_API_END
+}
+
+int mdft_server_set_batching(mdft_server s, uint32_t min_batch, double max_wait_us)
+{
+    MDFT_API_BEGIN
+    if (!s) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
+    if (min_batch < 1 || !(max_wait_us >= 0.0))
+        throw Error(MDFT_ERR_INVALID, "mdft_server_set_batching: min_batch >= 1 and max_wait_us >= 0");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->min_batch = std::min(min_batch, s->max_batch);
+    s->max_wait_us = max_wait_us;
     MDFT_API_END
 }
 

@@ -35,11 +35,26 @@ float time_launches(hipStream_t s, F&& fn, int reps)
     return ms / reps;
 }
 
-int tile_threads(int logm, bool wide)
+int tile_threads(int shape, bool wide)
 {
-    const int tpl = (1 << logm) / mdft::ept_of(logm);
+    const int tpl = mdft::m_of(shape) / mdft::ept_of(shape);
     const int base = wide ? 1024 : 256;
     return tpl > base ? tpl : base;
+}
+
+// every FFT shape of the tile kernels: powers of two, then 3 * 2^k (Bluestein only)
+std::vector<int> all_shapes()
+{
+    std::vector<int> v;
+    for (int logm = 1; logm <= mdft::MAX_LOGM; ++logm) v.push_back(logm);
+    for (int k = mdft::F3_MIN_K; k <= mdft::F3_MAX_K; ++k) v.push_back(mdft::F3_BASE + k);
+    return v;
+}
+
+bool valid_shape(int shape)
+{
+    return (shape >= 1 && shape <= mdft::MAX_LOGM) ||
+           (mdft::is_f3(shape) && mdft::f3_k(shape) >= mdft::F3_MIN_K && mdft::f3_k(shape) <= mdft::F3_MAX_K);
 }
 
 }  // namespace
@@ -59,14 +74,17 @@ int mdft_tune_auto(double* seconds)
     mdft::hip_check(hipMalloc(&buf, sizeof(float2) * BUF_ELEMS), "hipMalloc tuning buffer");
     mdft::hip_check(hipMemsetAsync(buf, 0, sizeof(float2) * BUF_ELEMS, s), "hipMemsetAsync");
     try {
-        for (int logm = 1; logm <= mdft::MAX_LOGM; ++logm) {
-            if (tile_threads(logm, false) == tile_threads(logm, true)) continue;  // one shape only
-            const int m = 1 << logm;
+        for (int shape : all_shapes()) {
+            if (tile_threads(shape, false) == tile_threads(shape, true)) continue;  // one tile size only
+            const int m = mdft::m_of(shape);
             for (int blue = 0; blue < 2; ++blue) {
-                if (blue && logm < 2) continue;
-                // a length with this padded length: m itself, or m/2 + 1 for Bluestein
-                const int n = blue ? m / 2 + 1 : m;
+                if ((blue && shape < 2) || (!blue && mdft::is_f3(shape))) continue;
+                // a length of this shape: m itself, m/2 + 1 for a power-of-two
+                // Bluestein convolution, m/2 for a 3 * 2^k one
+                const int n = !blue ? m : mdft::is_f3(shape) ? m / 2 : m / 2 + 1;
                 auto lp = ctx.line_plan(n);
+                if (lp->shape != shape) throw Error(MDFT_ERR_STATE, "tune: length " + std::to_string(n) +
+                                                                        " does not have the shape being tuned");
                 const int count = (int)(BUF_ELEMS / (size_t)n);
                 for (int cols = 0; cols < 2; ++cols) {
                     // rows: count lines of n; columns: an n-high image of count columns
@@ -77,7 +95,7 @@ int mdft_tune_auto(double* seconds)
                     for (int v = 0; v < 2; ++v) {
                         const int threads = v ? 1024 : 256;
                         auto launch = [&]() {
-                            mdft::hip_check(mdft::launch_axis(logm, j, blue != 0, cols != 0, threads, s),
+                            mdft::hip_check(mdft::launch_axis(shape, j, blue != 0, cols != 0, threads, s),
                                             "tuning launch");
                         };
                         const float ms = time_launches(s, launch, 5);
@@ -86,7 +104,7 @@ int mdft_tune_auto(double* seconds)
                             best_v = v;
                         }
                     }
-                    ctx.set_variant(logm, blue != 0, cols != 0, best_v);
+                    ctx.set_variant(shape, blue != 0, cols != 0, best_v);
                 }
             }
         }
@@ -110,11 +128,12 @@ int mdft_tune_save(const char* path)
     Context& ctx = Context::get(Context::current_device());
     FILE* f = std::fopen(path, "w");
     if (!f) throw Error(MDFT_ERR_INVALID, std::string("tune: cannot write ") + path);
-    std::fprintf(f, "# mdft tile shapes: logm blue cols variant (0 = 256-thread tiles, 1 = 1024-thread tiles)\n");
-    for (int logm = 1; logm <= mdft::MAX_LOGM; ++logm)
+    std::fprintf(f, "# mdft tile shapes: shape blue cols variant (0 = 256-thread tiles, 1 = 1024-thread tiles;\n"
+                    "# shape 1..14: 2^shape points, 38..43: 3 * 2^(shape - 32) points)\n");
+    for (int shape : all_shapes())
         for (int blue = 0; blue < 2; ++blue)
             for (int cols = 0; cols < 2; ++cols)
-                std::fprintf(f, "%d %d %d %d\n", logm, blue, cols, ctx.variant(logm, blue != 0, cols != 0));
+                std::fprintf(f, "%d %d %d %d\n", shape, blue, cols, ctx.variant(shape, blue != 0, cols != 0));
     const bool ok = std::fclose(f) == 0;
     if (!ok) throw Error(MDFT_ERR_INVALID, std::string("tune: cannot write ") + path);
     MDFT_API_END
@@ -137,8 +156,8 @@ int mdft_tune_load(const char* path)
         ++lineno;
         if (line[0] == '#' || line[0] == '\n') continue;
         Entry e{};
-        if (std::sscanf(line, "%d %d %d %d", &e.logm, &e.blue, &e.cols, &e.v) != 4 || e.logm < 1 ||
-            e.logm > mdft::MAX_LOGM || (e.blue != 0 && e.blue != 1) || (e.cols != 0 && e.cols != 1) ||
+        if (std::sscanf(line, "%d %d %d %d", &e.logm, &e.blue, &e.cols, &e.v) != 4 || !valid_shape(e.logm) ||
+            (e.blue != 0 && e.blue != 1) || (e.cols != 0 && e.cols != 1) ||
             (e.v != 0 && e.v != 1)) {
             bad = true;
             break;

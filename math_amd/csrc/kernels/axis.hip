@@ -11,72 +11,76 @@
 namespace mdft {
 
 // Workgroup sizes of the single-image kernels.  Narrow: 256 threads, or
-// one line when a line needs more (M = 8192 -> 512, M = 16384 -> 1024).
+// one line when a line needs more (M = 8192 or 6144 -> 512, M = 16384 -> 1024).
 // Wide: 1024 threads, so a tile holds 16384 points (e.g. 4 columns of 4096),
 // which widens the contiguous run a column tile reads from each row.
-__host__ __device__ constexpr int axis_threads(int logm, bool wide)
+__host__ __device__ constexpr int axis_threads(int shape, bool wide)
 {
-    const int tpl = (1 << logm) / ept_of(logm);
+    const int tpl = m_of(shape) / ept_of(shape);
     const int base = wide ? 1024 : 256;
     return tpl > base ? tpl : base;
 }
 
-template <int LOGM, int T, bool BLUE, bool COLS>
+template <int S, int T, bool BLUE, bool COLS>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_axis(AxisJob job)
 {
-    constexpr int LF = LineCfg<LOGM, T>::template lds_floats<COLS>();
+    constexpr int LF = LineCfg<S, T>::template lds_floats<COLS>();
     __shared__ float lds[LF > 0 ? LF : 1];
     const int tile = COLS ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-    line_tile<LOGM, T, BLUE, COLS>(job, tile, lds, threadIdx.x);
+    line_tile<S, T, BLUE, COLS>(job, tile, lds, threadIdx.x);
 }
 
-template <int LOGM, int T>
+template <int S, int T>
 static hipError_t launch_axis_tt(const AxisJob& job, bool blue, bool cols, hipStream_t stream)
 {
-    constexpr int LPW = LineCfg<LOGM, T>::LPW;
+    constexpr int LPW = LineCfg<S, T>::LPW;
     // in 64 bits: count may be up to 2^31 - 1
     const int64_t grid64 = (int64_t(job.count) + LPW - 1) / LPW;
     if (job.count < 0 || grid64 > 0x7fffffff) return hipErrorInvalidValue;
     if (grid64 == 0) return hipSuccess;
     const unsigned grid = (unsigned)grid64;
     if (blue) {
-        if constexpr (LOGM >= 2) {
+        if constexpr (S >= 2) {
             if (cols)
-                hipLaunchKernelGGL((k_axis<LOGM, T, true, true>), dim3(grid), dim3(T), 0, stream, job);
+                hipLaunchKernelGGL((k_axis<S, T, true, true>), dim3(grid), dim3(T), 0, stream, job);
             else
-                hipLaunchKernelGGL((k_axis<LOGM, T, true, false>), dim3(grid), dim3(T), 0, stream, job);
+                hipLaunchKernelGGL((k_axis<S, T, true, false>), dim3(grid), dim3(T), 0, stream, job);
         } else {
             return hipErrorInvalidValue;
         }
     } else {
-        if (cols)
-            hipLaunchKernelGGL((k_axis<LOGM, T, false, true>), dim3(grid), dim3(T), 0, stream, job);
-        else
-            hipLaunchKernelGGL((k_axis<LOGM, T, false, false>), dim3(grid), dim3(T), 0, stream, job);
+        if constexpr (!is_f3(S)) {
+            if (cols)
+                hipLaunchKernelGGL((k_axis<S, T, false, true>), dim3(grid), dim3(T), 0, stream, job);
+            else
+                hipLaunchKernelGGL((k_axis<S, T, false, false>), dim3(grid), dim3(T), 0, stream, job);
+        } else {
+            return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }
 
 // threads: 1024 = wide tiles, 256 = narrow, 128 / 64 = small tiles for
 // passes with few lines (clamped up to one line per workgroup)
-template <int LOGM>
+template <int S>
 static hipError_t launch_axis_t(const AxisJob& job, bool blue, bool cols, int threads, hipStream_t stream)
 {
-    constexpr int TPL = (1 << LOGM) / ept_of(LOGM);
-    constexpr int TN = axis_threads(LOGM, false), TW = axis_threads(LOGM, true);
-    if (threads >= 1024 && TW != TN) return launch_axis_tt<LOGM, TW>(job, blue, cols, stream);
+    constexpr int TPL = m_of(S) / ept_of(S);
+    constexpr int TN = axis_threads(S, false), TW = axis_threads(S, true);
+    if (threads >= 1024 && TW != TN) return launch_axis_tt<S, TW>(job, blue, cols, stream);
     if constexpr (TPL <= 64) {
-        if (threads <= 64) return launch_axis_tt<LOGM, 64>(job, blue, cols, stream);
+        if (threads <= 64) return launch_axis_tt<S, 64>(job, blue, cols, stream);
     }
     if constexpr (TPL <= 128) {
-        if (threads <= 128) return launch_axis_tt<LOGM, 128>(job, blue, cols, stream);
+        if (threads <= 128) return launch_axis_tt<S, 128>(job, blue, cols, stream);
     }
-    return launch_axis_tt<LOGM, TN>(job, blue, cols, stream);
+    return launch_axis_tt<S, TN>(job, blue, cols, stream);
 }
 
-hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, int threads, hipStream_t stream)
+hipError_t launch_axis(int shape, const AxisJob& job, bool blue, bool cols, int threads, hipStream_t stream)
 {
-    switch (logm) {
+    switch (shape) {
     case 1: return launch_axis_t<1>(job, blue, cols, threads, stream);
     case 2: return launch_axis_t<2>(job, blue, cols, threads, stream);
     case 3: return launch_axis_t<3>(job, blue, cols, threads, stream);
@@ -91,6 +95,12 @@ hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, int t
     case 12: return launch_axis_t<12>(job, blue, cols, threads, stream);
     case 13: return launch_axis_t<13>(job, blue, cols, threads, stream);
     case 14: return launch_axis_t<14>(job, blue, cols, threads, stream);
+    case F3_BASE + 6: return launch_axis_t<F3_BASE + 6>(job, blue, cols, threads, stream);
+    case F3_BASE + 7: return launch_axis_t<F3_BASE + 7>(job, blue, cols, threads, stream);
+    case F3_BASE + 8: return launch_axis_t<F3_BASE + 8>(job, blue, cols, threads, stream);
+    case F3_BASE + 9: return launch_axis_t<F3_BASE + 9>(job, blue, cols, threads, stream);
+    case F3_BASE + 10: return launch_axis_t<F3_BASE + 10>(job, blue, cols, threads, stream);
+    case F3_BASE + 11: return launch_axis_t<F3_BASE + 11>(job, blue, cols, threads, stream);
     default: return hipErrorInvalidValue;
     }
 }

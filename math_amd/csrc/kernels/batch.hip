@@ -37,14 +37,14 @@ __host__ __device__ constexpr int batch_maxlogm(int threads)
                : (threads >= 1024 ? 10 : threads >= 512 ? 9 : 8) + fft_bits(13);
 }
 
-template <int LOGM, int T>
+template <int S, int T>
 constexpr int lds_need()
 {
-    if constexpr ((1 << LOGM) / ept_of(LOGM) > T) {
+    if constexpr (m_of(S) / ept_of(S) > T) {
         return 0;
     } else {
-        constexpr int a = LineCfg<LOGM, T>::template lds_floats<false>();
-        constexpr int b = LineCfg<LOGM, T>::template lds_floats<true>();
+        constexpr int a = LineCfg<S, T>::template lds_floats<false>();
+        constexpr int b = LineCfg<S, T>::template lds_floats<true>();
         return a > b ? a : b;
     }
 }
@@ -54,60 +54,65 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 template <int T>
 constexpr int batch_lds()
 {
-    return cmax(cmax(cmax(lds_need<5, T>(), lds_need<6, T>()), cmax(lds_need<7, T>(), lds_need<8, T>())),
-                cmax(cmax(cmax(lds_need<9, T>(), lds_need<10, T>()), cmax(lds_need<11, T>(), lds_need<12, T>())),
-                     lds_need<13, T>()));
+    return cmax(cmax(cmax(cmax(lds_need<5, T>(), lds_need<6, T>()), cmax(lds_need<7, T>(), lds_need<8, T>())),
+                     cmax(cmax(cmax(lds_need<9, T>(), lds_need<10, T>()), cmax(lds_need<11, T>(), lds_need<12, T>())),
+                          lds_need<13, T>())),
+                cmax(cmax(cmax(lds_need<F3_BASE + 6, T>(), lds_need<F3_BASE + 7, T>()),
+                          cmax(lds_need<F3_BASE + 8, T>(), lds_need<F3_BASE + 9, T>())),
+                     cmax(lds_need<F3_BASE + 10, T>(), lds_need<F3_BASE + 11, T>())));
 }
 
-template <int LOGM, int T>
+template <int S, int T>
 __device__ __forceinline__ void run_tile_body(const AxisJob& job, int blue, int cols, int tile, float* lds)
 {
-    if constexpr ((1 << LOGM) / ept_of(LOGM) <= T) {
-        if constexpr (LOGM >= 2) {
+    if constexpr (m_of(S) / ept_of(S) <= T) {
+        if constexpr (S >= 2) {
             if (blue) {
                 if (cols)
-                    line_tile<LOGM, T, true, true>(job, tile, lds, threadIdx.x);
+                    line_tile<S, T, true, true>(job, tile, lds, threadIdx.x);
                 else
-                    line_tile<LOGM, T, true, false>(job, tile, lds, threadIdx.x);
+                    line_tile<S, T, true, false>(job, tile, lds, threadIdx.x);
                 return;
             }
         }
-        if (cols)
-            line_tile<LOGM, T, false, true>(job, tile, lds, threadIdx.x);
-        else
-            line_tile<LOGM, T, false, false>(job, tile, lds, threadIdx.x);
+        if constexpr (!is_f3(S)) {
+            if (cols)
+                line_tile<S, T, false, true>(job, tile, lds, threadIdx.x);
+            else
+                line_tile<S, T, false, false>(job, tile, lds, threadIdx.x);
+        }
     }
 }
 
 // Grid kernels inline every variant (register allocation per region,
 // occupancy-constrained); the persistent kernels call them out of line, since
 // inlining into the ticket loop keeps too much alive across it.
-template <int LOGM, int T, bool INL>
+template <int S, int T, bool INL>
 struct RunTile;
-template <int LOGM, int T>
-struct RunTile<LOGM, T, true> {
+template <int S, int T>
+struct RunTile<S, T, true> {
     __device__ __forceinline__ static void run(const AxisJob& job, int blue, int cols, int tile, float* lds)
     {
-        run_tile_body<LOGM, T>(job, blue, cols, tile, lds);
+        run_tile_body<S, T>(job, blue, cols, tile, lds);
     }
 };
-template <int LOGM, int T>
+template <int S, int T>
 __device__ __noinline__ void run_tile_call(const AxisJob& job, int blue, int cols, int tile, float* lds)
 {
-    run_tile_body<LOGM, T>(job, blue, cols, tile, lds);
+    run_tile_body<S, T>(job, blue, cols, tile, lds);
 }
-template <int LOGM, int T>
-struct RunTile<LOGM, T, false> {
+template <int S, int T>
+struct RunTile<S, T, false> {
     __device__ __forceinline__ static void run(const AxisJob& job, int blue, int cols, int tile, float* lds)
     {
-        run_tile_call<LOGM, T>(job, blue, cols, tile, lds);
+        run_tile_call<S, T>(job, blue, cols, tile, lds);
     }
 };
 
 template <int T, bool INL>
 __device__ __forceinline__ void run_job_tile(const BatchJob& bj, int tile, float* lds)
 {
-    switch (bj.logm) {
+    switch (bj.shape) {
     case 1: RunTile<1, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
     case 2: RunTile<2, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
     case 3: RunTile<3, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
@@ -121,6 +126,12 @@ __device__ __forceinline__ void run_job_tile(const BatchJob& bj, int tile, float
     case 11: RunTile<11, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
     case 12: RunTile<12, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
     case 13: RunTile<13, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
+    case F3_BASE + 6: RunTile<F3_BASE + 6, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
+    case F3_BASE + 7: RunTile<F3_BASE + 7, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
+    case F3_BASE + 8: RunTile<F3_BASE + 8, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
+    case F3_BASE + 9: RunTile<F3_BASE + 9, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
+    case F3_BASE + 10: RunTile<F3_BASE + 10, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
+    case F3_BASE + 11: RunTile<F3_BASE + 11, T, INL>::run(bj.job, bj.blue, bj.cols, tile, lds); break;
     default: break;
     }
 }
@@ -248,11 +259,13 @@ hipError_t launch_batch_persistent(int threads, const BatchJob* jobs, const uint
     return hipGetLastError();
 }
 
-int batch_lines_per_tile(int threads, int logm)
+int batch_lines_per_tile(int threads, int shape)
 {
-    if (logm < 1 || logm > batch_maxlogm(threads)) return 0;
-    const int tpl = (1 << logm) / ept_of(logm);
-    return threads / tpl;
+    if (shape < 1 || (!is_f3(shape) && shape > batch_maxlogm(threads)) ||
+        (is_f3(shape) && (f3_k(shape) < F3_MIN_K || f3_k(shape) > F3_MAX_K)))
+        return 0;
+    const int tpl = m_of(shape) / ept_of(shape);
+    return tpl <= threads ? threads / tpl : 0;
 }
 
 hipError_t batch_occupancy(int threads, int* blocks_per_cu)

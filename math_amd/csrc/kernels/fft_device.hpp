@@ -3,10 +3,11 @@
 // One "axis pass" transforms `count` lines of logical length n, laid out with
 // element stride es and line stride ls inside a row-major complex<float>
 // image (rows: es = 1, ls = w; columns: es = w, ls = 1).  A line of length n
-// is transformed through an FFT of length M = 2^LOGM: directly when n is a
-// power of two (M = n), otherwise by Bluestein's chirp-z method with M the
-// reference's padded length (src/gpu/dft/function.cpp:32-45) and the chirp
-// tables of function.cpp:49-92 generated in FP64 on the device.
+// is transformed through an FFT of length M: directly when n is a power of
+// two (M = n), otherwise by Bluestein's chirp-z method with M >= 2n - 1: the
+// reference's padded length bit_ceil(2n - 2) (src/gpu/dft/function.cpp:32-45),
+// or 3 * 2^k when that is shorter, and the chirp tables of function.cpp:49-92
+// generated in FP64 on the device.
 //
 // A workgroup owns LPW whole lines (a "tile").  Each thread holds EPT = 16
 // points of one line in VGPRs, at positions t + i*TPL (TPL = M/16 threads per
@@ -62,60 +63,93 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 // ---------------------------------------------------------------------------
 // compile-time line geometry
 // ---------------------------------------------------------------------------
-// Bits per radix pass of a line of padded length 2^logm: radix-16 passes (16
-// points per thread) or radix-8 passes (8 points per thread, half the
-// registers and LDS per thread).  A function of the length alone, so a line
-// runs the same schedule in every kernel.  16384-point lines keep radix 16 so
-// that one line fits a 1024-thread workgroup.
+// A line's FFT has a "shape" S:
+//  * S in [1, MAX_LOGM]: M = 2^S, radix-16 passes (16 points per thread) and
+//    one remainder pass, or radix-8 passes when built with MDFT_FFT_BITS=3;
+//  * S = F3_BASE + k: M = 3 * 2^k (Bluestein convolutions only), 12 points per
+//    thread: radix-4 passes, one radix-2 pass when k is odd, and a final
+//    radix-12 pass (a 3 x 4 prime-factor butterfly).
+// The shape is a function of the line length alone, so a line runs the same
+// schedule in every kernel.
 #ifndef MDFT_FFT_BITS
 #define MDFT_FFT_BITS 4
 #endif
 static_assert(MDFT_FFT_BITS == 3 || MDFT_FFT_BITS == 4, "radix-8 or radix-16 passes");
-__host__ __device__ constexpr int fft_bits(int logm) { return logm >= 14 ? 4 : MDFT_FFT_BITS; }
 // minimum waves per SIMD the tile kernels are compiled for (register cap 512 / waves)
 #ifndef MDFT_TILE_WAVES
 #define MDFT_TILE_WAVES 4
 #endif
-__host__ __device__ constexpr int ept_of(int logm)
+constexpr int MAX_LOGM = 14;
+constexpr int F3_BASE = 32;
+constexpr int F3_MIN_K = 6;   // M = 192: the radix-12 pass follows at least two radix-4 passes
+constexpr int F3_MAX_K = 11;  // M = 6144: 512 threads per line
+__host__ __device__ constexpr bool is_f3(int s) { return s >= F3_BASE; }
+__host__ __device__ constexpr int f3_k(int s) { return s - F3_BASE; }
+// 16384-point lines keep radix 16 so that one line fits a 1024-thread workgroup
+__host__ __device__ constexpr int fft_bits(int s) { return is_f3(s) ? 2 : s >= 14 ? 4 : MDFT_FFT_BITS; }
+__host__ __device__ constexpr int m_of(int s) { return is_f3(s) ? (3 << f3_k(s)) : (1 << s); }
+__host__ __device__ constexpr int ept_of(int s)
 {
-    return logm < fft_bits(logm) ? (1 << logm) : (1 << fft_bits(logm));
+    return is_f3(s) ? 12 : s < fft_bits(s) ? (1 << s) : (1 << fft_bits(s));
 }
-__host__ __device__ constexpr int npass_of(int logm) { return (logm + fft_bits(logm) - 1) / fft_bits(logm); }
-__host__ __device__ constexpr int pass_bits(int logm, int p)
+__host__ __device__ constexpr int npass_of(int s)
 {
-    return (logm - fft_bits(logm) * p) >= fft_bits(logm) ? fft_bits(logm) : (logm - fft_bits(logm) * p);
+    return is_f3(s) ? (f3_k(s) - 2) / 2 + (f3_k(s) - 2) % 2 + 1 : (s + fft_bits(s) - 1) / fft_bits(s);
 }
+// radix of pass p (1 when p is past the last pass)
+__host__ __device__ constexpr int radix_of(int s, int p)
+{
+    if (p >= npass_of(s)) return 1;
+    if (is_f3(s)) {
+        const int a = f3_k(s) - 2;
+        return p < a / 2 ? 4 : (p == npass_of(s) - 1 ? 12 : 2);
+    }
+    const int b = fft_bits(s);
+    return 1 << ((s - b * p) >= b ? b : (s - b * p));
+}
+// Stockham stride before pass p: the product of the earlier radices
+__host__ __device__ constexpr int ns_of(int s, int p)
+{
+    int ns = 1;
+    for (int q = 0; q < p; ++q) ns *= radix_of(s, q);
+    return ns;
+}
+// bits of the LDS padding block: one pad word per 2^padbits words
+__host__ __device__ constexpr int padbits_of(int s) { return is_f3(s) ? 4 : fft_bits(s); }
 
-// Twiddle rows of pass p >= 1 (Ns = (2^fft_bits)^p before the pass, radix R): row k
+// Twiddle rows of pass p >= 1 (stride Ns = ns_of(s, p), radix R): row k
 // (k < Ns) holds TW_ROW = 4 entries w^1, w^2, w^3, w^4 of
 // w = exp(-2*pi*i*k/(Ns*R)) (32 bytes, four rows per cache line); the other
 // powers are products of these (expand_twiddles).
 constexpr int TW_ROW = 4;
-__host__ __device__ constexpr int64_t tw_pass_len(int logm, int p)
+__host__ __device__ constexpr int64_t tw_pass_len(int s, int p)
 {
-    return (pass_bits(logm, p) > 0 ? (int64_t(1) << (fft_bits(logm) * p)) : 0) * TW_ROW;
+    return (p >= 1 && p < npass_of(s) ? int64_t(ns_of(s, p)) : 0) * TW_ROW;
 }
-__host__ __device__ constexpr int64_t tw_pass_off(int logm, int p)
+__host__ __device__ constexpr int64_t tw_pass_off(int s, int p)
 {
     int64_t off = 0;
-    for (int q = 1; q < p; ++q) off += tw_pass_len(logm, q);
+    for (int q = 1; q < p; ++q) off += tw_pass_len(s, q);
     return off;
 }
-__host__ __device__ constexpr int64_t tw_len(int logm)
+__host__ __device__ constexpr int64_t tw_len(int s)
 {
-    int64_t s = 0;
-    for (int p = 1; p < npass_of(logm); ++p) s += tw_pass_len(logm, p);
-    return s;
+    int64_t t = 0;
+    for (int p = 1; p < npass_of(s); ++p) t += tw_pass_len(s, p);
+    return t;
 }
-// All LOGM in [1, MAX_LOGM] share one table; offset of LOGM's block.
-constexpr int MAX_LOGM = 14;
-__host__ __device__ constexpr int64_t tw_off(int logm)
+// Every shape has a block of one table: powers of two 1..MAX_LOGM, then
+// 3 * 2^k for k in [F3_MIN_K, F3_MAX_K].
+__host__ __device__ constexpr int64_t tw_off(int s)
 {
-    int64_t s = 0;
-    for (int l = 1; l < logm; ++l) s += tw_len(l);
-    return s;
+    int64_t t = 0;
+    const int plast = is_f3(s) ? MAX_LOGM + 1 : s;
+    for (int l = 1; l < plast; ++l) t += tw_len(l);
+    if (is_f3(s))
+        for (int k = F3_MIN_K; k < f3_k(s); ++k) t += tw_len(F3_BASE + k);
+    return t;
 }
-__host__ __device__ constexpr int64_t tw_total() { return tw_off(MAX_LOGM + 1); }
+__host__ __device__ constexpr int64_t tw_total() { return tw_off(F3_BASE + F3_MAX_K + 1); }
 
 // LDS word of line position p in the exchange written by a pass of stride NS.
 // A pass writes its outputs NS positions apart and the next pass reads
@@ -124,16 +158,15 @@ __host__ __device__ constexpr int64_t tw_total() { return tw_off(MAX_LOGM + 1); 
 // unpadded and the writes need: NS = 1 (lanes 16 apart) one pad word per 32,
 // NS = 16 (lanes in two runs of 16, 256 apart) 16 pad words per 256, NS >= 256
 // (consecutive lanes) none; otherwise lanes span several lines and one pad
-// word per 16 with an odd line stride keeps them apart.  Radix-8 passes: one
-// pad word per 8.  Every map satisfies f(t + i*TPL) = f(t) + f(i*TPL) for
-// t < TPL, so LDS offsets are immediates.
-template <int NS, bool ONE_LINE, int B>
+// word per 2^PB (PB = padbits_of, 16 for radix-16 and radix-4/12 schedules, 8
+// for radix-8) with an odd line stride keeps them apart.  Every map satisfies
+// f(t + i*TPL) = f(t) + f(i*TPL) for t < TPL (TPL a power of two), so LDS
+// offsets are immediates.
+template <int NS, bool ONE_LINE, int PB>
 __host__ __device__ constexpr int xpad(int p)
 {
-    if constexpr (B == 3) {
-        return p + (p >> 3);
-    } else if constexpr (!ONE_LINE) {
-        return p + (p >> 4);
+    if constexpr (!ONE_LINE) {
+        return p + (p >> PB);
     } else if constexpr (NS == 1) {
         return p + (p >> 5);
     } else if constexpr (NS == 16) {
@@ -143,22 +176,33 @@ __host__ __device__ constexpr int xpad(int p)
     }
 }
 
-template <int LOGM, int THREADS>
+// A pass of radix R and stride NS writes its R outputs at base + r*NS; the
+// padded offsets advance by a constant step when the outputs of a butterfly
+// stay in one padding block (NS == 1, or NS * R within a block) or the
+// stride is a whole number of blocks.
+__host__ __device__ constexpr bool write_linear(int ns, int r, int pb)
+{
+    return ns == 1 ? ((1 << pb) % r == 0) : ns < (1 << pb) ? (ns * r <= (1 << pb) && (1 << pb) % (ns * r) == 0)
+                                                          : ns % (1 << pb) == 0;
+}
+
+template <int S, int THREADS>
 struct LineCfg {
-    static constexpr int M = 1 << LOGM;
-    static constexpr int EPT = ept_of(LOGM);
+    static constexpr int M = m_of(S);
+    static constexpr int EPT = ept_of(S);
     static constexpr int TPL = M / EPT;          // threads per line
     static constexpr int LPW = THREADS / TPL;    // lines per tile
-    static constexpr int NPASS = npass_of(LOGM);
+    static constexpr int NPASS = npass_of(S);
     static_assert(TPL <= THREADS, "line does not fit the workgroup");
     static_assert(LPW * TPL == THREADS, "workgroup must hold whole lines");
-    static constexpr int B = fft_bits(LOGM);
-    static constexpr int PADM = M + (M >> B);
-    // a 32-lane half-wave covers 32 consecutive positions of one line
+    static constexpr int PB = padbits_of(S);
+    static constexpr int PADM = M + (M >> PB);
+    // a 32-lane half-wave covers 32 consecutive positions of one line (the
+    // special layouts of radix-16 schedules)
     template <bool COLS>
     static constexpr bool one_line()
     {
-        return COLS ? LPW == 1 : TPL >= 32;
+        return PB != 4 || is_f3(S) ? false : COLS ? LPW == 1 : TPL >= 32;
     }
     // Line stride in LDS words.  Lanes of one wave cover q consecutive
     // positions of 32/q lines (rows: q = TPL; columns, lines fastest:
@@ -279,20 +323,52 @@ __device__ __forceinline__ void dif_stages(float2* u)
     }
 }
 
+// forward 3-point DFT in place, W = exp(-2*pi*i/3): X1 = t2 - i*t3,
+// X2 = t2 + i*t3 with t2 = x0 - (x1 + x2)/2, t3 = sin(pi/3) * (x1 - x2)
+__device__ __forceinline__ void dft3(float2& x0, float2& x1, float2& x2)
+{
+    const float h = 0.86602540378443864676f;
+    const float2 t1 = cadd(x1, x2);
+    const float2 t2 = make_float2(__builtin_fmaf(-0.5f, t1.x, x0.x), __builtin_fmaf(-0.5f, t1.y, x0.y));
+    const float2 t3 = make_float2(h * (x1.x - x2.x), h * (x1.y - x2.y));
+    x0 = cadd(x0, t1);
+    x1 = make_float2(t2.x + t3.y, t2.y - t3.x);
+    x2 = make_float2(t2.x - t3.y, t2.y + t3.x);
+}
+
 // ZHI: inputs u[R/2 + 1 ..] are zero (the zero-padded half of a Bluestein
-// line in its first pass)
+// line in its first pass).  R = 12 is the prime-factor (Good-Thomas) 3 x 4
+// DFT: no twiddles; input n = (4*n1 + 3*n2) mod 12, output k = (4*k1 + 9*k2)
+// mod 12 (k1 = k mod 3, k2 = k mod 4).
 template <int R, bool ZHI = false>
 __device__ __forceinline__ void dft_fwd(float2* u)
 {
-    static_assert(R >= 1 && R <= 16 && (R & (R - 1)) == 0, "radix");
-    if constexpr (R > 1) {
-        dif_stages<R, R / 2, ZHI>(u);
-        constexpr int bits = (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : 4;
-        float2 t[R];
+    if constexpr (R == 12) {
+        static_assert(!ZHI, "no zero-input pruning for radix 12 (never a first pass)");
+        float2 a0[4] = {u[0], u[3], u[6], u[9]};
+        float2 a1[4] = {u[4], u[7], u[10], u[1]};
+        float2 a2[4] = {u[8], u[11], u[2], u[5]};
+        dft_fwd<4>(a0);
+        dft_fwd<4>(a1);
+        dft_fwd<4>(a2);
 #pragma unroll
-        for (int i = 0; i < R; ++i) t[i] = u[bitrev_small(i, bits)];
+        for (int k2 = 0; k2 < 4; ++k2) {
+            dft3(a0[k2], a1[k2], a2[k2]);
+            u[(9 * k2) % 12] = a0[k2];
+            u[(4 + 9 * k2) % 12] = a1[k2];
+            u[(8 + 9 * k2) % 12] = a2[k2];
+        }
+    } else {
+        static_assert(R >= 1 && R <= 16 && (R & (R - 1)) == 0, "radix");
+        if constexpr (R > 1) {
+            dif_stages<R, R / 2, ZHI>(u);
+            constexpr int bits = (R == 2) ? 1 : (R == 4) ? 2 : (R == 8) ? 3 : 4;
+            float2 t[R];
 #pragma unroll
-        for (int i = 0; i < R; ++i) u[i] = t[i];
+            for (int i = 0; i < R; ++i) t[i] = u[bitrev_small(i, bits)];
+#pragma unroll
+            for (int i = 0; i < R; ++i) u[i] = t[i];
+        }
     }
 }
 
@@ -312,7 +388,7 @@ struct AxisJob {
     const float2* chirp;      // load: c (forward) or conj(c) (inverse), k < n
     const float2* dtab;       // middle: D or conj(D), D = FFT_M(conj(c) embedded) / M
     const float2* chirp_out;  // store: conj(c) (forward) or c/n (inverse), k < n
-    const float2* tw;      // twiddle block of this LOGM
+    const float2* tw;      // twiddle block of the line's FFT shape
 };
 
 // ---------------------------------------------------------------------------
@@ -379,10 +455,10 @@ __device__ __forceinline__ void exchange_sync()
 // ahead (pass 1's with the tile's input), so the table latency overlaps the
 // LDS exchange instead of following it.  Off for 16384 points, where the
 // extra registers spill.
-template <int LOGM>
+template <int S>
 __host__ __device__ constexpr bool tw_prefetch()
 {
-    return LOGM <= 13;
+    return is_f3(S) || S <= 13;
 }
 
 // float4s of a row a radix-R butterfly reads: (w^1, w^2) and (w^3, w^4)
@@ -392,20 +468,20 @@ __host__ __device__ constexpr int tw_quads()
     return R <= 2 ? 1 : 2;
 }
 
-template <int LOGM, int THREADS, int P>
+template <int S, int THREADS, int P>
 struct TwLoad {
-    using C = LineCfg<LOGM, THREADS>;
-    static constexpr int R = 1 << pass_bits(LOGM, P);
+    using C = LineCfg<S, THREADS>;
+    static constexpr int R = radix_of(S, P);
     static constexpr int Q = tw_quads<R>();
     static constexpr int NB = C::EPT / R;
-    static constexpr int NS = 1 << (C::B * P);
+    static constexpr int NS = ns_of(S, P);
     static_assert(NB * Q <= C::EPT / 2, "twiddle registers");
     __device__ __forceinline__ static void run(float4 (&w)[C::EPT / 2], int t, const float2* tw)
     {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const int k = (t + b * C::TPL) & (NS - 1);
-            const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * TW_ROW);
+            const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(S, P) + int64_t(k) * TW_ROW);
 #pragma unroll
             for (int r2 = 0; r2 < Q; ++r2) w[b * Q + r2] = row[r2];
         }
@@ -413,8 +489,20 @@ struct TwLoad {
 };
 
 // the R twiddles of one butterfly (wt[0] unused) from its row: w^1..w^4
-// loaded, w^(4a+b) = w^(4a) * w^b, w^8 = w^4 * w^4, w^12 = w^8 * w^4 (at most
-// four roundings from the FP64-generated entries)
+// loaded, w^(4a+b) = w^(4a) * w^b, w^(4a) = w^(4a-4) * w^4 (at most four
+// roundings from the FP64-generated entries)
+template <int R, int r>
+__device__ __forceinline__ void expand_products(float2 (&wt)[R])
+{
+    if constexpr (r < R) {
+        if constexpr (r % 4 == 0)
+            wt[r] = cmul(wt[r - 4], wt[4]);
+        else
+            wt[r] = cmul(wt[r - r % 4], wt[r % 4]);
+        expand_products<R, r + 1>(wt);
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void expand_twiddles(float2 (&wt)[R], const float4* q)
 {
@@ -425,17 +513,7 @@ __device__ __forceinline__ void expand_twiddles(float2 (&wt)[R], const float4* q
     }
     if constexpr (R > 4) {
         wt[4] = make_float2(q[1].z, q[1].w);
-#pragma unroll
-        for (int b = 1; b < 4; ++b) wt[4 + b] = cmul(wt[4], wt[b]);
-    }
-    if constexpr (R > 8) {
-        wt[8] = cmul(wt[4], wt[4]);
-        wt[12] = cmul(wt[8], wt[4]);
-#pragma unroll
-        for (int b = 1; b < 4; ++b) {
-            wt[8 + b] = cmul(wt[8], wt[b]);
-            wt[12 + b] = cmul(wt[12], wt[b]);
-        }
+        expand_products<R, 5>(wt);
     }
 }
 
@@ -444,18 +522,17 @@ __device__ __forceinline__ void expand_twiddles(float2 (&wt)[R], const float4* q
 // loaded into pf one pass before the last, so their latency overlaps the
 // last exchange and butterflies; slots PF .. PFL-1 at the start of the last
 // pass, after its exchange (with one pass, all PFL at its start).
-template <int LOGM, int THREADS, bool COLS, int P, bool ZIN = false, int PF = 0, int PFL = 0>
+template <int S, int THREADS, bool COLS, int P, bool ZIN = false, int PF = 0, int PFL = 0>
 struct FftPass {
-    using C = LineCfg<LOGM, THREADS>;
+    using C = LineCfg<S, THREADS>;
     static constexpr bool WAVE = !COLS && C::TPL <= 64;
-    static constexpr int RB = pass_bits(LOGM, P);
-    static constexpr int R = 1 << RB;
+    static constexpr int R = radix_of(S, P);
     static constexpr int NB = C::EPT / R;
-    static constexpr int NS = 1 << (C::B * P);
-    static constexpr int RADIX = 1 << C::B;
+    static constexpr int NS = ns_of(S, P);
     static constexpr bool LAST = (P == C::NPASS - 1);
     static constexpr int LST = C::template lstride<COLS>();
     static constexpr bool ONE = C::template one_line<COLS>();
+    static_assert(NB * R == C::EPT, "the radix divides the points per thread");
 
     __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, float* re, float* im, const float2* tw,
                                                float4 (&w)[C::EPT / 2], const float2* pfs, float2 (&pf)[C::EPT])
@@ -467,12 +544,13 @@ struct FftPass {
         }
         if constexpr (P > 0) {
             exchange_sync<WAVE>();
-            // layout of the previous pass (stride NS/16); the offset of slot i
-            // is a compile-time constant (an LDS immediate)
-            const int q0 = xpad<NS / RADIX, ONE, C::B>(t);
+            // layout written by the previous pass; the offset of slot i is a
+            // compile-time constant (an LDS immediate)
+            constexpr int NSP = ns_of(S, P - 1);
+            const int q0 = xpad<NSP, ONE, C::PB>(t);
 #pragma unroll
             for (int i = 0; i < C::EPT; ++i) {
-                const int q = q0 + xpad<NS / RADIX, ONE, C::B>(i * C::TPL);
+                const int q = q0 + xpad<NSP, ONE, C::PB>(i * C::TPL);
                 v[i] = make_float2(re[q], im[q]);
             }
             if constexpr (LAST && PFL > PF) {
@@ -491,21 +569,22 @@ struct FftPass {
             if constexpr (P > 0) {
                 constexpr int Q = tw_quads<R>();
                 float4 q[2];
-                const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(LOGM, P) + int64_t(k) * TW_ROW);
+                const float4* row = reinterpret_cast<const float4*>(tw + tw_pass_off(S, P) + int64_t(k) * TW_ROW);
 #pragma unroll
-                for (int r2 = 0; r2 < Q; ++r2) q[r2] = tw_prefetch<LOGM>() ? w[b * Q + r2] : row[r2];
+                for (int r2 = 0; r2 < Q; ++r2) q[r2] = tw_prefetch<S>() ? w[b * Q + r2] : row[r2];
                 float2 wt[R];
                 expand_twiddles<R>(wt, q);
 #pragma unroll
                 for (int r = 1; r < R; ++r) u[r] = cmul(u[r], wt[r]);
             }
-            static_assert(!ZIN || (P == 0 && NB == 1), "zero inputs only in a single-butterfly first pass");
+            // ZIN: slot b + NB*r holds a zero for r > R/2 (b + NB*(R/2 + 1) >= EPT/2 + 1)
+            static_assert(!ZIN || P == 0, "zero inputs only in the first pass");
             dft_fwd<R, ZIN>(u);
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b + NB * r] = u[r];
         }
         if constexpr (!LAST) {
-            if constexpr (tw_prefetch<LOGM>()) TwLoad<LOGM, THREADS, P + 1>::run(w, t, tw);
+            if constexpr (tw_prefetch<S>()) TwLoad<S, THREADS, P + 1>::run(w, t, tw);
             if constexpr (PF > 0 && P + 2 == C::NPASS) {
 #pragma unroll
                 for (int i = 0; i < PF; ++i) pf[i] = pfs[t + i * C::TPL];
@@ -516,13 +595,12 @@ struct FftPass {
                 const int j = t + b * C::TPL;
                 const int k = j & (NS - 1);
                 const int base = (j - k) * R + k;
-                // NS == 1 (first pass, full radix): base is a multiple of the
-                // radix, so the outputs are contiguous after padding; NS >= the
-                // radix: the outputs lie in one padding block apart from whole
-                // blocks, so each step adds xpad(NS)
-                static_assert(NS == 1 ? R == RADIX : NS % RADIX == 0, "linear write offsets");
-                constexpr int step = xpad<NS, ONE, C::B>(NS);
-                const int q0 = xpad<NS, ONE, C::B>(base);
+                // the R outputs stay in one padding block, or the stride is a
+                // whole number of blocks: each step adds xpad(NS)
+                static_assert(ONE ? (NS == 1 ? R == 16 : NS % 16 == 0) : write_linear(NS, R, C::PB),
+                              "linear write offsets");
+                constexpr int step = xpad<NS, ONE, C::PB>(NS);
+                const int q0 = xpad<NS, ONE, C::PB>(base);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int q = q0 + r * step;
@@ -530,7 +608,7 @@ struct FftPass {
                     im[q] = v[b + NB * r].y;
                 }
             }
-            FftPass<LOGM, THREADS, COLS, P + 1, false, PF, PFL>::run(v, t, re, im, tw, w, pfs, pf);
+            FftPass<S, THREADS, COLS, P + 1, false, PF, PFL>::run(v, t, re, im, tw, w, pfs, pf);
         }
     }
 };
@@ -540,9 +618,9 @@ struct FftPass {
 // constant zero).  PF > 0: pf[i] = pfs[t + i*TPL] for i < max(PF, PFL) on
 // return, loaded during the FFT (pfs must be readable at every such
 // position).
-template <int LOGM, int THREADS, bool COLS, bool ZIN = false, int PF = 0, int PFL = 0>
+template <int S, int THREADS, bool COLS, bool ZIN = false, int PF = 0, int PFL = 0>
 struct FftTile {
-    using C = LineCfg<LOGM, THREADS>;
+    using C = LineCfg<S, THREADS>;
     __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, int ll, float* lds, const float2* tw)
     {
         static_assert(PF == 0 && PFL == 0, "prefetch needs a destination");
@@ -556,8 +634,8 @@ struct FftTile {
         float* re = lds + ll * LST;
         float* im = lds + C::LPW * LST + ll * LST;
         float4 w[C::EPT / 2];
-        if constexpr (tw_prefetch<LOGM>() && C::NPASS > 1) TwLoad<LOGM, THREADS, 1>::run(w, t, tw);
-        FftPass<LOGM, THREADS, COLS, 0, ZIN, PF, PFL>::run(v, t, re, im, tw, w, pfs, pf);
+        if constexpr (tw_prefetch<S>() && C::NPASS > 1) TwLoad<S, THREADS, 1>::run(w, t, tw);
+        FftPass<S, THREADS, COLS, 0, ZIN, PF, PFL>::run(v, t, re, im, tw, w, pfs, pf);
     }
 };
 
@@ -567,10 +645,11 @@ struct FftTile {
 // IO (image kernels only): IO_LUM loads luminance from io.img instead of
 // complex values from job.src; IO_MAG stores |X| * io.mscale to io.mag
 // instead of complex values to job.dst.
-template <int LOGM, int THREADS, bool BLUE, bool COLS, int IO = 0>
+template <int S, int THREADS, bool BLUE, bool COLS, int IO = 0>
 __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* lds, int tid, ImageIO io = ImageIO{})
 {
-    using C = LineCfg<LOGM, THREADS>;
+    static_assert(BLUE || !is_f3(S), "3 * 2^k lengths are Bluestein convolutions");
+    using C = LineCfg<S, THREADS>;
     const int ll = COLS ? (tid % C::LPW) : (tid / C::TPL);
     const int t = COLS ? (tid / C::LPW) : (tid % C::TPL);
     const int line = tile * C::LPW + ll;
@@ -586,9 +665,12 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
     // a power-of-two line fills the FFT (n == M); only Bluestein pads
     auto in_line = [&](int pos) { return !BLUE || pos < n; };
     // Bluestein: n <= M/2 + 1, so slot i holds a position of the line only
-    // for i <= EPT/2 (slot EPT/2 only at t = 0 when n = M/2 + 1); the other
-    // slots are zero on input and not stored on output.
-    constexpr int LIVE = BLUE ? C::EPT / 2 + 1 : C::EPT;
+    // for i <= EPT/2 (slot EPT/2 only at t = 0 when n = M/2 + 1; for 3 * 2^k
+    // lengths M >= 2n - 1, so i < EPT/2); the other slots are zero on input
+    // and not stored on output.
+    constexpr int LIVE = BLUE ? (is_f3(S) ? C::EPT / 2 : C::EPT / 2 + 1) : C::EPT;
+    // the first FFT skips the zero half in the butterflies of its first pass
+    constexpr bool ZIN = BLUE;
 
     float2 v[C::EPT];
     {
@@ -616,7 +698,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
     constexpr int PFD = (BLUE && C::NPASS <= 3) ? C::EPT / 2 : 0;
     constexpr int PFDL = PFD > 0 ? C::EPT : 0;
     float2 pf[C::EPT];
-    FftTile<LOGM, THREADS, COLS, BLUE, PFD, PFDL>::run(v, t, ll, lds, job.tw, job.dtab, pf);
+    FftTile<S, THREADS, COLS, ZIN, PFD, PFDL>::run(v, t, ll, lds, job.tw, job.dtab, pf);
 
     if constexpr (BLUE) {
         // forward conj(v*D), inverse conj(v)*D = conj(v*conj(D)); the outer
@@ -628,7 +710,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
         }
         // the output chirp block is followed by readable entries up to
         // position M/2 + M/16 (dir_tables_len), so every live slot can be loaded
-        FftTile<LOGM, THREADS, COLS, false, LIVE>::run(v, t, ll, lds, job.tw, job.chirp_out, pf);
+        FftTile<S, THREADS, COLS, false, LIVE>::run(v, t, ll, lds, job.tw, job.chirp_out, pf);
     }
 
     if (valid) {

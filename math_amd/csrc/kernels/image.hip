@@ -13,37 +13,40 @@
 
 namespace mdft {
 
-__host__ __device__ constexpr int image_threads(int logm, bool wide)
+__host__ __device__ constexpr int image_threads(int shape, bool wide)
 {
-    const int tpl = (1 << logm) / ept_of(logm);
+    const int tpl = m_of(shape) / ept_of(shape);
     const int base = wide ? 1024 : 256;
     return tpl > base ? tpl : base;
 }
 
-template <int LOGM, int T, bool BLUE, bool COLS, int IO>
+template <int S, int T, bool BLUE, bool COLS, int IO>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_image_axis(AxisJob job, ImageIO io)
 {
-    constexpr int LF = LineCfg<LOGM, T>::template lds_floats<COLS>();
+    constexpr int LF = LineCfg<S, T>::template lds_floats<COLS>();
     __shared__ float lds[LF > 0 ? LF : 1];
     const int tile = COLS ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-    line_tile<LOGM, T, BLUE, COLS, IO>(job, tile, lds, threadIdx.x, io);
+    line_tile<S, T, BLUE, COLS, IO>(job, tile, lds, threadIdx.x, io);
 }
 
-template <int LOGM, int T, bool COLS, int IO>
+template <int S, int T, bool COLS, int IO>
 static hipError_t launch_image_tt(const AxisJob& job, const ImageIO& io, bool blue, hipStream_t stream)
 {
-    constexpr int LPW = LineCfg<LOGM, T>::LPW;
+    constexpr int LPW = LineCfg<S, T>::LPW;
     const int64_t grid64 = (int64_t(job.count) + LPW - 1) / LPW;
     if (job.count < 0 || grid64 > 0x7fffffff) return hipErrorInvalidValue;
     if (grid64 == 0) return hipSuccess;
     const unsigned grid = (unsigned)grid64;
     if (blue) {
-        if constexpr (LOGM >= 2)
-            hipLaunchKernelGGL((k_image_axis<LOGM, T, true, COLS, IO>), dim3(grid), dim3(T), 0, stream, job, io);
+        if constexpr (S >= 2)
+            hipLaunchKernelGGL((k_image_axis<S, T, true, COLS, IO>), dim3(grid), dim3(T), 0, stream, job, io);
         else
             return hipErrorInvalidValue;
     } else {
-        hipLaunchKernelGGL((k_image_axis<LOGM, T, false, COLS, IO>), dim3(grid), dim3(T), 0, stream, job, io);
+        if constexpr (!is_f3(S))
+            hipLaunchKernelGGL((k_image_axis<S, T, false, COLS, IO>), dim3(grid), dim3(T), 0, stream, job, io);
+        else
+            return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -51,28 +54,28 @@ static hipError_t launch_image_tt(const AxisJob& job, const ImageIO& io, bool bl
 // rows: IO_LUM, or IO_LUM|IO_MAG for a one-row image; columns: IO_MAG, or
 // IO_LUM|IO_MAG for a one-column image.  Wide tiles only for the column
 // pass that stores magnitudes (the shape of the ordinary column pass).
-template <int LOGM>
+template <int S>
 static hipError_t launch_image_t(const AxisJob& job, const ImageIO& io, bool blue, bool cols, int mode, bool wide,
                                  hipStream_t stream)
 {
-    constexpr int TN = image_threads(LOGM, false), TW = image_threads(LOGM, true);
+    constexpr int TN = image_threads(S, false), TW = image_threads(S, true);
     if (!cols) {
-        if (mode == IO_LUM) return launch_image_tt<LOGM, TN, false, IO_LUM>(job, io, blue, stream);
-        if (mode == (IO_LUM | IO_MAG)) return launch_image_tt<LOGM, TN, false, IO_LUM | IO_MAG>(job, io, blue, stream);
+        if (mode == IO_LUM) return launch_image_tt<S, TN, false, IO_LUM>(job, io, blue, stream);
+        if (mode == (IO_LUM | IO_MAG)) return launch_image_tt<S, TN, false, IO_LUM | IO_MAG>(job, io, blue, stream);
         return hipErrorInvalidValue;
     }
     if (mode == IO_MAG) {
-        if (wide && TW != TN) return launch_image_tt<LOGM, TW, true, IO_MAG>(job, io, blue, stream);
-        return launch_image_tt<LOGM, TN, true, IO_MAG>(job, io, blue, stream);
+        if (wide && TW != TN) return launch_image_tt<S, TW, true, IO_MAG>(job, io, blue, stream);
+        return launch_image_tt<S, TN, true, IO_MAG>(job, io, blue, stream);
     }
-    if (mode == (IO_LUM | IO_MAG)) return launch_image_tt<LOGM, TN, true, IO_LUM | IO_MAG>(job, io, blue, stream);
+    if (mode == (IO_LUM | IO_MAG)) return launch_image_tt<S, TN, true, IO_LUM | IO_MAG>(job, io, blue, stream);
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_image_axis(int logm, const AxisJob& job, const ImageIO& io, bool blue, bool cols, int mode,
+hipError_t launch_image_axis(int shape, const AxisJob& job, const ImageIO& io, bool blue, bool cols, int mode,
                              bool wide, hipStream_t stream)
 {
-    switch (logm) {
+    switch (shape) {
     case 1: return launch_image_t<1>(job, io, blue, cols, mode, wide, stream);
     case 2: return launch_image_t<2>(job, io, blue, cols, mode, wide, stream);
     case 3: return launch_image_t<3>(job, io, blue, cols, mode, wide, stream);
@@ -87,6 +90,12 @@ hipError_t launch_image_axis(int logm, const AxisJob& job, const ImageIO& io, bo
     case 12: return launch_image_t<12>(job, io, blue, cols, mode, wide, stream);
     case 13: return launch_image_t<13>(job, io, blue, cols, mode, wide, stream);
     case 14: return launch_image_t<14>(job, io, blue, cols, mode, wide, stream);
+    case F3_BASE + 6: return launch_image_t<F3_BASE + 6>(job, io, blue, cols, mode, wide, stream);
+    case F3_BASE + 7: return launch_image_t<F3_BASE + 7>(job, io, blue, cols, mode, wide, stream);
+    case F3_BASE + 8: return launch_image_t<F3_BASE + 8>(job, io, blue, cols, mode, wide, stream);
+    case F3_BASE + 9: return launch_image_t<F3_BASE + 9>(job, io, blue, cols, mode, wide, stream);
+    case F3_BASE + 10: return launch_image_t<F3_BASE + 10>(job, io, blue, cols, mode, wide, stream);
+    case F3_BASE + 11: return launch_image_t<F3_BASE + 11>(job, io, blue, cols, mode, wide, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -11,7 +11,7 @@ namespace mdft {
 // One axis pass of one image inside a batch.
 struct BatchJob {
     AxisJob job;
-    int logm;
+    int shape;  // FFT shape of the job's lines (fft_device.hpp)
     int blue;
     int cols;
     int ntiles;
@@ -68,11 +68,11 @@ hipError_t gen_dir_diagonal(float2* dir, const float2* dtab, int n, int m, hipSt
 hipError_t gen_dtab(float2* dtab, const double2* chirp64, double* cos_scratch, int n, int m, hipStream_t stream);
 
 // threads: 1024 wide tiles, 256 narrow, 128 or 64 small (clamped to one line per workgroup)
-hipError_t launch_axis(int logm, const AxisJob& job, bool blue, bool cols, int threads, hipStream_t stream);
+hipError_t launch_axis(int shape, const AxisJob& job, bool blue, bool cols, int threads, hipStream_t stream);
 
 // image path: an axis pass with fused luminance input and/or magnitude output
 // (mode = IO_LUM | IO_MAG bits), and the unfused conversions
-hipError_t launch_image_axis(int logm, const AxisJob& job, const ImageIO& io, bool blue, bool cols, int mode,
+hipError_t launch_image_axis(int shape, const AxisJob& job, const ImageIO& io, bool blue, bool cols, int mode,
                              bool wide, hipStream_t stream);
 hipError_t launch_image_luminance(const ImageIO& io, float2* dst, int w, int h, hipStream_t stream);
 hipError_t launch_image_magnitude(const ImageIO& io, const float2* src, int w, int h, hipStream_t stream);
@@ -86,7 +86,8 @@ hipError_t launch_batch_grid(int threads, const BatchJob* jobs, const uint2* ite
                              hipStream_t stream);
 hipError_t launch_batch_persistent(int threads, const BatchJob* jobs, const uint2* items, uint32_t nitems,
                                    uint32_t* ctl, uint32_t nctl, int grid, uint32_t spin_limit, hipStream_t stream);
-int batch_lines_per_tile(int threads, int logm);
+// lines of a shape one tile of `threads` threads holds (0: the shape does not fit)
+int batch_lines_per_tile(int threads, int shape);
 hipError_t batch_occupancy(int threads, int* blocks_per_cu);
 int batch_max_logm(int threads);
 

@@ -133,16 +133,18 @@ hipError_t gen_h2_f32(float2* h2, const double2* chirp64, int64_t n, int64_t m, 
 
 hipError_t gen_twiddles(float2* table, hipStream_t stream)
 {
-    for (int logm = 1; logm <= MAX_LOGM; ++logm) {
-        for (int p = 1; p < npass_of(logm); ++p) {
-            const int ns = 1 << (fft_bits(logm) * p);
-            const int radix = 1 << pass_bits(logm, p);
+    auto shape_rows = [&](int shape) {
+        for (int p = 1; p < npass_of(shape); ++p) {
+            const int ns = ns_of(shape, p);
+            const int radix = radix_of(shape, p);
             const int64_t len = int64_t(ns) * TW_ROW;
-            float2* out = table + tw_off(logm) + tw_pass_off(logm, p);
+            float2* out = table + tw_off(shape) + tw_pass_off(shape, p);
             const int blocks = (int)((len + 255) / 256);
             hipLaunchKernelGGL(k_gen_twiddle_block, dim3(blocks), dim3(256), 0, stream, out, ns, radix);
         }
-    }
+    };
+    for (int logm = 1; logm <= MAX_LOGM; ++logm) shape_rows(logm);
+    for (int k = F3_MIN_K; k <= F3_MAX_K; ++k) shape_rows(F3_BASE + k);
     return hipGetLastError();
 }
 

@@ -133,6 +133,7 @@ def lib():
             "mdft_batch_destroy": ([_vp], ctypes.c_int),
             "mdft_server_create": ([ctypes.POINTER(_vp), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32],
                                    ctypes.c_int),
+            "mdft_server_set_batching": ([_vp, ctypes.c_uint32, ctypes.c_double], ctypes.c_int),
             "mdft_server_prepare": ([_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32], ctypes.c_int),
             "mdft_server_submit": ([_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp,
                                     ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
@@ -182,14 +183,14 @@ def cache_set_limit(nbytes):
     _check(lib().mdft_cache_set_limit(int(nbytes)))
 
 
-def tune_get(logm, blue, cols):
+def tune_get(shape, blue, cols):
     v = ctypes.c_int()
-    _check(lib().mdft_tune_get(logm, int(blue), int(cols), ctypes.byref(v)))
+    _check(lib().mdft_tune_get(shape, int(blue), int(cols), ctypes.byref(v)))
     return v.value
 
 
-def tune_set(logm, blue, cols, variant):
-    _check(lib().mdft_tune_set(logm, int(blue), int(cols), int(variant)))
+def tune_set(shape, blue, cols, variant):
+    _check(lib().mdft_tune_set(shape, int(blue), int(cols), int(variant)))
 
 
 def tune_auto():
@@ -401,6 +402,10 @@ class Server:
     def __init__(self, budget_bytes, streams=0, max_batch=0):
         self._h = _vp()
         _check(lib().mdft_server_create(ctypes.byref(self._h), int(budget_bytes), int(streams), int(max_batch)))
+
+    def set_batching(self, min_batch, max_wait_us):
+        """Batching window of micro-batches launched while another runs (mdft_server_set_batching)."""
+        _check(lib().mdft_server_set_batching(self._h, int(min_batch), float(max_wait_us)))
 
     def prepare(self, lengths):
         arr = (ctypes.c_uint32 * max(len(lengths), 1))(*[int(n) for n in lengths])

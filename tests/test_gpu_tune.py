@@ -18,8 +18,12 @@ def mdft():
     return M
 
 
+# FFT shapes: 2^s points for s in 1..14, 3 * 2^(s - 32) points for s in 38..43
+SHAPES = list(range(1, 15)) + list(range(38, 44))
+
+
 def table(M):
-    return {(l, b, c): M.tune_get(l, b, c) for l in range(1, 15) for b in (0, 1) for c in (0, 1)}
+    return {(l, b, c): M.tune_get(l, b, c) for l in SHAPES for b in (0, 1) for c in (0, 1)}
 
 
 def test_autotune_save_load(mdft, tmp_path):

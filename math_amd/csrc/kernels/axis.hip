@@ -22,7 +22,7 @@ __host__ __device__ constexpr int axis_threads(int shape, bool wide)
 }
 
 template <int S, int T, bool BLUE, bool COLS>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_axis(AxisJob job)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(TILE_WAVES))) void k_axis(AxisJob job)
 {
     constexpr int LF = LineCfg<S, T>::template lds_floats<COLS>();
     __shared__ float lds[LF > 0 ? LF : 1];

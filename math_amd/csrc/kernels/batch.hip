@@ -159,7 +159,7 @@ __device__ __forceinline__ void run_item(const BatchJob* __restrict__ jobs, uint
 // Four waves per SIMD (<= 128 VGPRs): 256-thread tiles (4 workgroups per
 // CU), 512-thread tiles (two) or 1024-thread tiles (one).
 template <int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_batch_grid(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(TILE_WAVES))) void k_batch_grid(
     const BatchJob* __restrict__ jobs, const uint2* __restrict__ items)
 {
     __shared__ float lds[batch_lds<T>()];
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAV
 // Deadlock freedom of the dependency mode: an item waits only on items with
 // smaller tickets, taken by running workgroups that never wait on later ones.
 template <int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_batch_persistent(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(TILE_WAVES))) void k_batch_persistent(
     const BatchJob* __restrict__ jobs, const uint2* __restrict__ items, uint32_t nitems, uint32_t* ctl,
     uint32_t spin_limit)
 {

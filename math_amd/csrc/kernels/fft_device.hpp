@@ -65,28 +65,22 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 // ---------------------------------------------------------------------------
 // A line's FFT has a "shape" S:
 //  * S in [1, MAX_LOGM]: M = 2^S, radix-16 passes (16 points per thread) and
-//    one remainder pass, or radix-8 passes when built with MDFT_FFT_BITS=3;
+//    one remainder pass;
 //  * S = F3_BASE + k: M = 3 * 2^k (Bluestein convolutions only), 12 points per
 //    thread: radix-4 passes, one radix-2 pass when k is odd, and a final
 //    radix-12 pass (a 3 x 4 prime-factor butterfly).
 // The shape is a function of the line length alone, so a line runs the same
 // schedule in every kernel.
-#ifndef MDFT_FFT_BITS
-#define MDFT_FFT_BITS 4
-#endif
-static_assert(MDFT_FFT_BITS == 3 || MDFT_FFT_BITS == 4, "radix-8 or radix-16 passes");
-// minimum waves per SIMD the tile kernels are compiled for (register cap 512 / waves)
-#ifndef MDFT_TILE_WAVES
-#define MDFT_TILE_WAVES 4
-#endif
 constexpr int MAX_LOGM = 14;
+// waves per SIMD the tile kernels are compiled for (at most 128 VGPRs)
+constexpr int TILE_WAVES = 4;
 constexpr int F3_BASE = 32;
 constexpr int F3_MIN_K = 6;   // M = 192: the radix-12 pass follows at least two radix-4 passes
 constexpr int F3_MAX_K = 11;  // M = 6144: 512 threads per line
 __host__ __device__ constexpr bool is_f3(int s) { return s >= F3_BASE; }
 __host__ __device__ constexpr int f3_k(int s) { return s - F3_BASE; }
-// 16384-point lines keep radix 16 so that one line fits a 1024-thread workgroup
-__host__ __device__ constexpr int fft_bits(int s) { return is_f3(s) ? 2 : s >= 14 ? 4 : MDFT_FFT_BITS; }
+// bits per pass of the full-radix passes
+__host__ __device__ constexpr int fft_bits(int s) { return is_f3(s) ? 2 : 4; }
 __host__ __device__ constexpr int m_of(int s) { return is_f3(s) ? (3 << f3_k(s)) : (1 << s); }
 __host__ __device__ constexpr int ept_of(int s)
 {
@@ -114,8 +108,6 @@ __host__ __device__ constexpr int ns_of(int s, int p)
     for (int q = 0; q < p; ++q) ns *= radix_of(s, q);
     return ns;
 }
-// bits of the LDS padding block: one pad word per 2^padbits words
-__host__ __device__ constexpr int padbits_of(int s) { return is_f3(s) ? 4 : fft_bits(s); }
 
 // Twiddle rows of pass p >= 1 (stride Ns = ns_of(s, p), radix R): row k
 // (k < Ns) holds TW_ROW = 4 entries w^1, w^2, w^3, w^4 of
@@ -151,39 +143,31 @@ __host__ __device__ constexpr int64_t tw_off(int s)
 }
 __host__ __device__ constexpr int64_t tw_total() { return tw_off(F3_BASE + F3_MAX_K + 1); }
 
-// LDS word of line position p in the exchange written by a pass of stride NS.
-// A pass writes its outputs NS positions apart and the next pass reads
-// positions TPL apart.  Radix-16 passes: when a 32-lane half-wave covers 32
-// consecutive positions of one line (ONE_LINE), the reads are conflict-free
-// unpadded and the writes need: NS = 1 (lanes 16 apart) one pad word per 32,
-// NS = 16 (lanes in two runs of 16, 256 apart) 16 pad words per 256, NS >= 256
-// (consecutive lanes) none; otherwise lanes span several lines and one pad
-// word per 2^PB (PB = padbits_of, 16 for radix-16 and radix-4/12 schedules, 8
-// for radix-8) with an odd line stride keeps them apart.  Every map satisfies
-// f(t + i*TPL) = f(t) + f(i*TPL) for t < TPL (TPL a power of two), so LDS
-// offsets are immediates.
-template <int NS, bool ONE_LINE, int PB>
-__host__ __device__ constexpr int xpad(int p)
+// LDS layout of the exchange written by pass P: line position p is stored at
+// word p + c * (p >> sh) of the line's slice, with (c, sh) chosen per FFT
+// shape, workgroup size, axis and pass by gen_lds_maps.py (lds_maps.inc) so
+// that (a) offsets stay linear, f(t + i*TPL) = f(t) + f(i*TPL) for the next
+// pass's reads and f(base + r*NS) = f(base) + r*f(NS) for this pass's writes,
+// making every LDS access a base register plus an immediate, and (b) no
+// write or read instruction has a bank conflict (32 banks, 32 lanes per cycle).
+struct LdsMap {
+    int shape, threads, cols, pass, c, sh;
+};
+constexpr LdsMap kLdsMaps[] = {
+#include "lds_maps.inc"
+};
+__host__ __device__ constexpr LdsMap lds_map(int shape, int threads, bool cols, int pass)
 {
-    if constexpr (!ONE_LINE) {
-        return p + (p >> PB);
-    } else if constexpr (NS == 1) {
-        return p + (p >> 5);
-    } else if constexpr (NS == 16) {
-        return p + 16 * (p >> 8);
-    } else {
-        return p;
-    }
+    for (const LdsMap& e : kLdsMaps)
+        if (e.shape == shape && e.threads == threads && e.cols == (cols ? 1 : 0) && e.pass == pass) return e;
+    return LdsMap{shape, threads, cols ? 1 : 0, pass, -1, 0};  // missing: rejected by a static_assert
 }
-
-// A pass of radix R and stride NS writes its R outputs at base + r*NS; the
-// padded offsets advance by a constant step when the outputs of a butterfly
-// stay in one padding block (NS == 1, or NS * R within a block) or the
-// stride is a whole number of blocks.
-__host__ __device__ constexpr bool write_linear(int ns, int r, int pb)
+template <int S, int THREADS, bool COLS, int P>
+__host__ __device__ constexpr int lds_pos(int p)
 {
-    return ns == 1 ? ((1 << pb) % r == 0) : ns < (1 << pb) ? (ns * r <= (1 << pb) && (1 << pb) % (ns * r) == 0)
-                                                          : ns % (1 << pb) == 0;
+    constexpr LdsMap m = lds_map(S, THREADS, COLS, P);
+    static_assert(m.c >= 0, "no LDS layout generated for this shape, workgroup size and pass");
+    return p + m.c * (p >> m.sh);
 }
 
 template <int S, int THREADS>
@@ -195,25 +179,29 @@ struct LineCfg {
     static constexpr int NPASS = npass_of(S);
     static_assert(TPL <= THREADS, "line does not fit the workgroup");
     static_assert(LPW * TPL == THREADS, "workgroup must hold whole lines");
-    static constexpr int PB = padbits_of(S);
-    static constexpr int PADM = M + (M >> PB);
-    // a 32-lane half-wave covers 32 consecutive positions of one line (the
-    // special layouts of radix-16 schedules)
-    template <bool COLS>
-    static constexpr bool one_line()
+    // words of one line's slice: the largest padded position over the exchanges
+    template <bool COLS, int P = 0>
+    static constexpr int padm()
     {
-        return PB != 4 || is_f3(S) ? false : COLS ? LPW == 1 : TPL >= 32;
+        if constexpr (P + 1 >= NPASS) {
+            return M;
+        } else {
+            const int a = lds_pos<S, THREADS, COLS, P>(M - 1) + 1;
+            const int b = padm<COLS, P + 1>();
+            return a > b ? a : b;
+        }
     }
     // Line stride in LDS words.  Lanes of one wave cover q consecutive
     // positions of 32/q lines (rows: q = TPL; columns, lines fastest:
     // q = 32/LPW); a stride of q times an odd number puts them on distinct
-    // banks.
+    // banks (the same rule gen_lds_maps.py assumes).
     template <bool COLS>
     static constexpr int lstride()
     {
         const int q = COLS ? (LPW >= 32 ? 1 : 32 / LPW) : (TPL >= 32 ? 0 : TPL);
-        if (q == 0) return PADM;
-        int s = PADM;
+        const int pm = padm<COLS>();
+        if (q == 0) return pm;
+        int s = pm;
         while (s % (2 * q) != q) ++s;
         return s;
     }
@@ -531,7 +519,6 @@ struct FftPass {
     static constexpr int NS = ns_of(S, P);
     static constexpr bool LAST = (P == C::NPASS - 1);
     static constexpr int LST = C::template lstride<COLS>();
-    static constexpr bool ONE = C::template one_line<COLS>();
     static_assert(NB * R == C::EPT, "the radix divides the points per thread");
 
     __device__ __forceinline__ static void run(float2 (&v)[C::EPT], int t, float* re, float* im, const float2* tw,
@@ -546,11 +533,10 @@ struct FftPass {
             exchange_sync<WAVE>();
             // layout written by the previous pass; the offset of slot i is a
             // compile-time constant (an LDS immediate)
-            constexpr int NSP = ns_of(S, P - 1);
-            const int q0 = xpad<NSP, ONE, C::PB>(t);
+            const int q0 = lds_pos<S, THREADS, COLS, P - 1>(t);
 #pragma unroll
             for (int i = 0; i < C::EPT; ++i) {
-                const int q = q0 + xpad<NSP, ONE, C::PB>(i * C::TPL);
+                const int q = q0 + lds_pos<S, THREADS, COLS, P - 1>(i * C::TPL);
                 v[i] = make_float2(re[q], im[q]);
             }
             if constexpr (LAST && PFL > PF) {
@@ -595,12 +581,9 @@ struct FftPass {
                 const int j = t + b * C::TPL;
                 const int k = j & (NS - 1);
                 const int base = (j - k) * R + k;
-                // the R outputs stay in one padding block, or the stride is a
-                // whole number of blocks: each step adds xpad(NS)
-                static_assert(ONE ? (NS == 1 ? R == 16 : NS % 16 == 0) : write_linear(NS, R, C::PB),
-                              "linear write offsets");
-                constexpr int step = xpad<NS, ONE, C::PB>(NS);
-                const int q0 = xpad<NS, ONE, C::PB>(base);
+                // the layout keeps the R outputs' offsets linear: each step adds f(NS)
+                constexpr int step = lds_pos<S, THREADS, COLS, P>(NS);
+                const int q0 = lds_pos<S, THREADS, COLS, P>(base);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int q = q0 + r * step;

@@ -21,7 +21,7 @@ __host__ __device__ constexpr int image_threads(int shape, bool wide)
 }
 
 template <int S, int T, bool BLUE, bool COLS, int IO>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(MDFT_TILE_WAVES))) void k_image_axis(AxisJob job, ImageIO io)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(TILE_WAVES))) void k_image_axis(AxisJob job, ImageIO io)
 {
     constexpr int LF = LineCfg<S, T>::template lds_floats<COLS>();
     __shared__ float lds[LF > 0 ? LF : 1];

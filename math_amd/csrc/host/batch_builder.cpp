@@ -32,10 +32,8 @@ int batch_threads_for(const uint32_t* widths, const uint32_t* heights, size_t co
             throw Error(MDFT_ERR_SIZE, "batch transform " + std::to_string(i) + " (" + std::to_string(widths[i]) + "x" +
                                            std::to_string(heights[i]) + ") exceeds the batch padded-length limit " +
                                            std::to_string(1 << limit));
-        for (uint32_t n : {widths[i], heights[i]}) {
-            const int s = line_shape(n);
-            tpl = std::max(tpl, m_of(s) / ept_of(s));
-        }
+        const int sr = line_shape(widths[i], false), sc = line_shape(heights[i], true);
+        tpl = std::max({tpl, m_of(sr) / ept_of(sr), m_of(sc) / ept_of(sc)});
     }
     // the smallest tile that holds a whole line of every length
     return tpl <= 256 ? 256 : tpl <= 512 ? 512 : 1024;
@@ -70,8 +68,8 @@ void build_batch(Context& ctx, const BatchEntry* entries, size_t count, int T, i
     for (uint32_t idx : order) {
         const BatchEntry& e = entries[idx];
         const int w = (int)e.width, h = (int)e.height;
-        auto rp = ctx.line_plan(w);
-        auto cp = ctx.line_plan(h);
+        auto rp = ctx.line_plan(w, false);
+        auto cp = ctx.line_plan(h, true);
         out.keep.push_back(rp);
         out.keep.push_back(cp);
         const int lpw_r = w > 1 ? batch_lines_per_tile(T, rp->shape) : 0;

@@ -57,8 +57,8 @@ int mdft_plan_create(mdft_plan* plan, uint32_t width, uint32_t height)
     p->device = ctx.device();
     p->w = width;
     p->h = height;
-    p->row = ctx.line_plan((int)width);
-    p->col = ctx.line_plan((int)height);
+    p->row = ctx.line_plan((int)width, false);
+    p->col = ctx.line_plan((int)height, true);
     p->scratch_elems = std::max(width > 1 ? p->row->scratch_elems((int)height) : uint64_t(0),
                                 height > 1 ? p->col->scratch_elems((int)width) : uint64_t(0));
     if (p->scratch_elems) {

@@ -44,7 +44,7 @@ int mdft_fft_create(mdft_fft* fft, uint32_t count, uint32_t n)
     f->count = count;
     f->n = n;
     if (n > 1) {
-        f->line = ctx.line_plan((int)n);
+        f->line = ctx.line_plan((int)n, false);
         const uint64_t se = f->line->scratch_elems((int)count);
         if (se) {
             mdft::DeviceGuard g(f->device);

@@ -44,8 +44,8 @@ int mdft_image_create(mdft_image* image, uint32_t width, uint32_t height, uint32
     p->w = width;
     p->h = height;
     p->fmt = pixel_format;
-    p->row = ctx.line_plan((int)width);
-    p->col = ctx.line_plan((int)height);
+    p->row = ctx.line_plan((int)width, false);
+    p->col = ctx.line_plan((int)height, true);
     p->scratch_elems = std::max(width > 1 ? p->row->scratch_elems((int)height) : uint64_t(0),
                                 height > 1 ? p->col->scratch_elems((int)width) : uint64_t(0));
     mdft::DeviceGuard g(p->device);

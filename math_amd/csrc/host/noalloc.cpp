@@ -89,8 +89,8 @@ int mdft_exec_device_noalloc(uint32_t width, uint32_t height, int inverse, const
     if (!ctx || !ptr)
         throw Error(MDFT_ERR_STATE, "mdft_exec_device_noalloc: device " + std::to_string(device) +
                                         " is not initialised (call mdft_init or create a plan first)");
-    auto row = ctx->find_line_plan((int)width);
-    auto col = ctx->find_line_plan((int)height);
+    auto row = ctx->find_line_plan((int)width, false);
+    auto col = ctx->find_line_plan((int)height, true);
     if (!row || !col)
         throw Error(MDFT_ERR_STATE, "mdft_exec_device_noalloc: the tables of length " +
                                         std::to_string(!row ? width : height) +

@@ -37,10 +37,10 @@ int compute_m(long long n)
     return (int)m;
 }
 
-int line_shape(long long n)
+int line_shape(long long n, bool cols)
 {
     const int m = compute_m(n);
-    if (is_pow2((uint64_t)n) || m > (1 << MAX_LOGM)) return ilog2((uint64_t)m);
+    if (!cols || is_pow2((uint64_t)n) || m > (1 << MAX_LOGM)) return ilog2((uint64_t)m);
     long long m3 = 3LL << F3_MIN_K;
     while (m3 < 2 * n - 1) m3 <<= 1;
     if (m3 < m && m3 <= (3LL << F3_MAX_K)) return F3_BASE + ilog2((uint64_t)(m3 / 3));
@@ -149,11 +149,14 @@ Context::Context(int device) : device_(device)
     }
 }
 
-std::shared_ptr<const LinePlan> Context::line_plan(int n)
+static int64_t line_key(int n, int shape) { return (int64_t(shape) << 32) | uint32_t(n); }
+
+std::shared_ptr<const LinePlan> Context::line_plan(int n, bool cols)
 {
     const int m = compute_m(n);  // validates n
+    const int shape = line_shape(n, cols);
     std::lock_guard<std::mutex> lk(mu_);
-    auto it = lines_.find(n);
+    auto it = lines_.find(line_key(n, shape));
     if (it != lines_.end()) {
         it->second.last_use = ++use_clock_;
         return it->second.plan;
@@ -161,7 +164,7 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
 
     auto lp = std::make_shared<LinePlan>();
     lp->n = n;
-    lp->shape = line_shape(n);
+    lp->shape = shape;
     lp->padded = m;
     lp->m = is_f3(lp->shape) ? m_of(lp->shape) : m;
     lp->logm = is_f3(lp->shape) ? 0 : ilog2((uint64_t)m);
@@ -229,15 +232,17 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n)
         }
     }
     hip_check(hipStreamSynchronize(gen_stream_), "table generation");
-    lines_[n] = CacheEntry{lp, ++use_clock_};
+    lines_[line_key(n, shape)] = CacheEntry{lp, ++use_clock_};
     trim_locked(cache_limit_);
     return lp;
 }
 
-std::shared_ptr<const LinePlan> Context::find_line_plan(int n)
+std::shared_ptr<const LinePlan> Context::find_line_plan(int n, bool cols)
 {
+    if (n < 1 || n > (1 << 29)) return nullptr;
+    const int shape = line_shape(n, cols);
     std::lock_guard<std::mutex> lk(mu_);
-    auto it = lines_.find(n);
+    auto it = lines_.find(line_key(n, shape));
     if (it == lines_.end()) return nullptr;
     it->second.last_use = ++use_clock_;
     return it->second.plan;

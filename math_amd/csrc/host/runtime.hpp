@@ -33,11 +33,12 @@ void hip_check(hipError_t e, const char* what);
 int compute_m(long long n);
 int ilog2(uint64_t v);
 bool is_pow2(uint64_t v);
-// FFT shape (fft_device.hpp) of a line of length n: log2(n) for a power of
-// two; for other lengths the Bluestein convolution length, the reference's
-// compute_m(n) = 2^j or 3 * 2^k >= 2n - 1 when that is shorter (3 * 2^k in
-// [192, 6144]); lines longer than one tile keep compute_m.
-int line_shape(long long n);
+// FFT shape (fft_device.hpp) of a line of length n along an axis: log2(n)
+// for a power of two; for other lengths the Bluestein convolution length, the
+// reference's compute_m(n) = 2^j, or along columns 3 * 2^k >= 2n - 1 when that
+// is shorter (3 * 2^k in [192, 6144]; measured faster for column tiles and
+// slower for row tiles); lines longer than one tile keep compute_m.
+int line_shape(long long n, bool cols);
 // index of a shape in per-shape tables, 0 .. NSHAPES - 1
 constexpr int NSHAPES = MAX_LOGM + 1 + (F3_MAX_K - F3_MIN_K + 1);
 inline int shape_index(int shape) { return is_f3(shape) ? MAX_LOGM + 1 + (f3_k(shape) - F3_MIN_K) : shape; }
@@ -47,7 +48,7 @@ struct LinePlan {
     int n = 0;
     int m = 1;         // FFT length: n for a power of two, the Bluestein convolution length otherwise
     int logm = 0;      // log2(m) for power-of-two m (lines longer than one tile: the four-step split)
-    int shape = 0;     // FFT shape of the tile kernels (line_shape(n))
+    int shape = 0;     // FFT shape of the tile kernels (line_shape(n, axis))
     int padded = 1;    // the reference's padded length compute_m(n) (reported by mdft_plan_info)
     bool blue = false;  // Bluestein (n not a power of two)
     float2* chirp = nullptr;  // n entries, Bluestein only
@@ -90,9 +91,9 @@ public:
     // Cached, device-resident plan of one line length (thread-safe).  The
     // cache holds at most cache_limit() bytes of tables that no caller holds:
     // beyond it the least recently requested unused lengths are freed.
-    std::shared_ptr<const LinePlan> line_plan(int n);
+    std::shared_ptr<const LinePlan> line_plan(int n, bool cols);
     // the cached plan of length n, or null; builds and allocates nothing
-    std::shared_ptr<const LinePlan> find_line_plan(int n);
+    std::shared_ptr<const LinePlan> find_line_plan(int n, bool cols);
     uint64_t cached_bytes();
     uint32_t cached_lengths();
     uint64_t cache_limit();
@@ -140,7 +141,8 @@ private:
         std::shared_ptr<const LinePlan> plan;
         uint64_t last_use = 0;
     };
-    std::unordered_map<int, CacheEntry> lines_;
+    // keyed by length and FFT shape (a length has at most two: compute_m and 3 * 2^k)
+    std::unordered_map<int64_t, CacheEntry> lines_;
     uint64_t use_clock_ = 0;
     uint64_t cache_limit_ = uint64_t(4) << 30;
     void trim_locked(uint64_t limit);

@@ -136,9 +136,9 @@ struct mdft_server_s {
     std::condition_variable cv_work, cv_done;
     std::deque<uint64_t> pending;
     std::unordered_map<uint64_t, Request> reqs;
-    // line plans of every length the server has seen: resident for its
-    // lifetime and counted against the budget with the result slots
-    std::unordered_map<int, std::shared_ptr<const mdft::LinePlan>> plans;
+    // line plans of every (length, FFT shape) the server has seen: resident
+    // for its lifetime and counted against the budget with the result slots
+    std::unordered_map<int64_t, std::shared_ptr<const mdft::LinePlan>> plans;
     uint64_t table_bytes = 0;
     std::vector<hipEvent_t> free_events;
     uint64_t next_id = 1;
@@ -155,8 +155,12 @@ size_t table_bytes(size_t jobs, size_t items) { return sizeof(BatchJob) * jobs +
 
 uint64_t slot_bytes(uint64_t bytes) { return (bytes + 255) & ~uint64_t(255); }
 
-// Makes the plans of a request's lengths resident (built on the calling
-// thread, not the scheduler's) and returns the table bytes they add.
+int64_t plan_key(int n, int shape) { return (int64_t(shape) << 32) | uint32_t(n); }
+
+// Makes the plans of a request's lines resident (built on the calling thread,
+// not the scheduler's) and returns the table bytes they add.  Line i has
+// length lengths[i] and runs along rows when i is even, along columns when
+// it is odd (a length can have a different FFT shape on each axis).
 uint64_t hold_plans(mdft_server_s* s, const uint32_t* lengths, size_t count,
                     std::vector<std::shared_ptr<const mdft::LinePlan>>& out)
 {
@@ -164,14 +168,16 @@ uint64_t hold_plans(mdft_server_s* s, const uint32_t* lengths, size_t count,
     uint64_t add = 0;
     for (size_t i = 0; i < count; ++i) {
         const int n = (int)lengths[i];
+        const bool cols = i & 1;
+        const int64_t key = plan_key(n, mdft::line_shape(n, cols));
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            if (s->plans.count(n)) continue;
+            if (s->plans.count(key)) continue;
         }
         bool dup = false;
-        for (auto& p : out) dup |= p->n == n;
+        for (auto& p : out) dup |= plan_key(p->n, p->shape) == key;
         if (dup) continue;
-        out.push_back(ctx.line_plan(n));
+        out.push_back(ctx.line_plan(n, cols));
         add += out.back()->bytes;
     }
     return add;
@@ -180,7 +186,7 @@ uint64_t hold_plans(mdft_server_s* s, const uint32_t* lengths, size_t count,
 void commit_plans(mdft_server_s* s, std::vector<std::shared_ptr<const mdft::LinePlan>>& held)
 {
     for (auto& p : held)
-        if (s->plans.emplace(p->n, p).second) s->table_bytes += p->bytes;
+        if (s->plans.emplace(plan_key(p->n, p->shape), p).second) s->table_bytes += p->bytes;
 }
 
 void grow(Lane& L, size_t njobs, size_t nitems)
@@ -383,8 +389,16 @@ int mdft_server_prepare(mdft_server s, const uint32_t* lengths, uint32_t count)
     MDFT_API_BEGIN
     if (!s) throw Error(MDFT_ERR_NO_BUFFERS, "No DFT buffers");
     if (!lengths && count) throw Error(MDFT_ERR_INVALID, "mdft_server_prepare: lengths is NULL");
+    // each length along both axes
+    std::vector<uint32_t> both;
+    both.reserve(2 * (size_t)count);
+    for (uint32_t i = 0; i < count; ++i) {
+        mdft::compute_m(lengths[i]);  // validates the length
+        both.push_back(lengths[i]);
+        both.push_back(lengths[i]);
+    }
     std::vector<std::shared_ptr<const mdft::LinePlan>> held;
-    hold_plans(s, lengths, count, held);
+    hold_plans(s, both.data(), both.size(), held);
     std::lock_guard<std::mutex> lk(s->mu);
     commit_plans(s, held);
     MDFT_API_END

@@ -82,7 +82,9 @@ int mdft_tune_auto(double* seconds)
                 // a length of this shape: m itself, m/2 + 1 for a power-of-two
                 // Bluestein convolution, m/2 for a 3 * 2^k one
                 const int n = !blue ? m : mdft::is_f3(shape) ? m / 2 : m / 2 + 1;
-                auto lp = ctx.line_plan(n);
+                // 3 * 2^k convolutions are only chosen for columns; the tile
+                // kernels of a shape are tuned along both axes all the same
+                auto lp = ctx.line_plan(n, mdft::is_f3(shape));
                 if (lp->shape != shape) throw Error(MDFT_ERR_STATE, "tune: length " + std::to_string(n) +
                                                                         " does not have the shape being tuned");
                 const int count = (int)(BUF_ELEMS / (size_t)n);

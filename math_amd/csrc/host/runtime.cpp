@@ -40,10 +40,11 @@ int compute_m(long long n)
 int line_shape(long long n, bool cols)
 {
     const int m = compute_m(n);
-    if (!cols || is_pow2((uint64_t)n) || m > (1 << MAX_LOGM)) return ilog2((uint64_t)m);
+    if (is_pow2((uint64_t)n) || m > (1 << MAX_LOGM)) return ilog2((uint64_t)m);
     long long m3 = 3LL << F3_MIN_K;
     while (m3 < 2 * n - 1) m3 <<= 1;
-    if (m3 < m && m3 <= (3LL << F3_MAX_K)) return F3_BASE + ilog2((uint64_t)(m3 / 3));
+    const long long min3 = cols ? (3LL << F3_MIN_K) : (3LL << F3_ROW_MIN_K);
+    if (m3 < m && m3 >= min3 && m3 <= (3LL << F3_MAX_K)) return F3_BASE + ilog2((uint64_t)(m3 / 3));
     return ilog2((uint64_t)m);
 }
 

@@ -35,9 +35,11 @@ int ilog2(uint64_t v);
 bool is_pow2(uint64_t v);
 // FFT shape (fft_device.hpp) of a line of length n along an axis: log2(n)
 // for a power of two; for other lengths the Bluestein convolution length, the
-// reference's compute_m(n) = 2^j, or along columns 3 * 2^k >= 2n - 1 when that
-// is shorter (3 * 2^k in [192, 6144]; measured faster for column tiles and
-// slower for row tiles); lines longer than one tile keep compute_m.
+// reference's compute_m(n) = 2^j, or 3 * 2^k >= 2n - 1 when that is shorter:
+// along columns for 3 * 2^k in [192, 6144], along rows in [3072, 6144] (the
+// shorter row tiles measured slower with the extra passes, the longer ones
+// faster); lines longer than one tile keep compute_m.
+constexpr int F3_ROW_MIN_K = 10;
 int line_shape(long long n, bool cols);
 // index of a shape in per-shape tables, 0 .. NSHAPES - 1
 constexpr int NSHAPES = MAX_LOGM + 1 + (F3_MAX_K - F3_MIN_K + 1);

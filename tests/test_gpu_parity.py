@@ -107,8 +107,9 @@ def test_all_line_lengths_1_to_300(mdft):
         check_against_fp64(got, xt, True, "cols")
 
 
-@pytest.mark.parametrize("n", [511, 512, 513, 1000, 1023, 1024, 1025, 2047, 2048, 2049, 4093, 4096, 4097, 8191,
-                               8192, 8193])
+# 1025-1536 and 2049-3072 (and 2500) run 3 * 2^k convolutions along both axes, 3073 the next power of two
+@pytest.mark.parametrize("n", [511, 512, 513, 1000, 1023, 1024, 1025, 1536, 2047, 2048, 2049, 2500, 3072, 3073,
+                               4093, 4096, 4097, 8191, 8192, 8193])
 def test_long_lines(mdft, n):
     rng = np.random.default_rng(n)
     x = rand_c64(rng, (2, n))

@@ -116,8 +116,9 @@ int mdft_exec_device(mdft_plan plan, int inverse, const void* src, void* dst, vo
  * allocate device memory (code under stream capture, tight serving loops,
  * harnesses that forbid hipMalloc); no counterpart in the reference.  The
  * same kernels transform a width x height device image without a plan
- * object: the chirp, diagonal and twiddle tables of both sides must already
- * be resident (keep a plan, batch or server of those lengths alive), and
+ * object: the tables of the width as a row length and of the height as a
+ * column length must already be resident (keep a plan of this size, or a
+ * batch or server that holds a transform of this size, alive), and
  * lines whose padded length exceeds 16384 use a static per-device scratch of
  * 32 MiB (4 Mi complex<float>) instead of plan-owned memory.  Calls no
  * hipMalloc, hipFree or other allocating routine.  Errors: MDFT_ERR_SIZE when

@@ -241,8 +241,9 @@ typedef struct mdft_batch_s* mdft_batch;
                                      this batch's column phase on scratch images at create time */
 
 /* Plan a batch of transforms (no counterpart in the reference, whose exec
- * handles one image per call).  Every side must have a padded length
- * compute_m(side) <= 8192: powers of two up to 8192, other sides up to 4097. */
+ * handles one image per call).  Every side must run an FFT of at most 8192
+ * points: powers of two up to 8192, 6144 (3 * 2^11, a direct FFT), other
+ * sides up to 4097 (padded length compute_m(side) <= 8192). */
 int mdft_batch_create(mdft_batch* batch, const mdft_desc* descs, uint32_t count, uint32_t flags);
 
 /* Enqueue the whole batch on a stream.  Default shape: two launches (small

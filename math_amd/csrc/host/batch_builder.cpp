@@ -27,12 +27,14 @@ int batch_threads_for(const uint32_t* widths, const uint32_t* heights, size_t co
     const int limit = batch_max_logm(1024);
     int tpl = 1;
     for (size_t i = 0; i < count; ++i) {
-        const int l1 = ilog2(compute_m(widths[i])), l2 = ilog2(compute_m(heights[i]));
-        if (l1 > limit || l2 > limit)
+        compute_m(widths[i]);  // validates the sides
+        compute_m(heights[i]);
+        // the FFT lengths the tiles run (the padded length, or 3 * 2^k)
+        const int sr = line_shape(widths[i], false), sc = line_shape(heights[i], true);
+        if (m_of(sr) > (1 << limit) || m_of(sc) > (1 << limit))
             throw Error(MDFT_ERR_SIZE, "batch transform " + std::to_string(i) + " (" + std::to_string(widths[i]) + "x" +
                                            std::to_string(heights[i]) + ") exceeds the batch padded-length limit " +
                                            std::to_string(1 << limit));
-        const int sr = line_shape(widths[i], false), sc = line_shape(heights[i], true);
         tpl = std::max({tpl, m_of(sr) / ept_of(sr), m_of(sc) / ept_of(sc)});
     }
     // the smallest tile that holds a whole line of every length

@@ -41,6 +41,9 @@ int line_shape(long long n, bool cols)
 {
     const int m = compute_m(n);
     if (is_pow2((uint64_t)n) || m > (1 << MAX_LOGM)) return ilog2((uint64_t)m);
+    // n = 3 * 2^k itself: a direct FFT of that shape, no Bluestein
+    if (n % 3 == 0 && is_pow2((uint64_t)(n / 3)) && n >= (3LL << F3_MIN_K) && n <= (3LL << F3_MAX_K))
+        return F3_BASE + ilog2((uint64_t)(n / 3));
     long long m3 = 3LL << F3_MIN_K;
     while (m3 < 2 * n - 1) m3 <<= 1;
     const long long min3 = cols ? (3LL << F3_MIN_K) : (3LL << F3_ROW_MIN_K);
@@ -169,7 +172,8 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n, bool cols)
     lp->padded = m;
     lp->m = is_f3(lp->shape) ? m_of(lp->shape) : m;
     lp->logm = is_f3(lp->shape) ? 0 : ilog2((uint64_t)m);
-    lp->blue = !is_pow2((uint64_t)n);
+    // powers of two and n = 3 * 2^k run a direct FFT of length n
+    lp->blue = lp->m != n;
     if (lp->logm > LARGE_MAX_LOGM)
         throw Error(MDFT_ERR_SIZE, "Error size " + std::to_string(n) + ": padded length " + std::to_string(m) +
                                        " exceeds the longest supported line (2^" + std::to_string(LARGE_MAX_LOGM) +

@@ -34,7 +34,8 @@ int compute_m(long long n);
 int ilog2(uint64_t v);
 bool is_pow2(uint64_t v);
 // FFT shape (fft_device.hpp) of a line of length n along an axis: log2(n)
-// for a power of two; for other lengths the Bluestein convolution length, the
+// for a power of two, 3 * 2^k for n = 3 * 2^k in [192, 6144] (both direct
+// FFTs of length n); for other lengths the Bluestein convolution length, the
 // reference's compute_m(n) = 2^j, or 3 * 2^k >= 2n - 1 when that is shorter:
 // along columns for 3 * 2^k in [192, 6144], along rows in [3072, 6144] (the
 // shorter row tiles measured slower with the extra passes, the longer ones

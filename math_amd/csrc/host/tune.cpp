@@ -42,7 +42,7 @@ int tile_threads(int shape, bool wide)
     return tpl > base ? tpl : base;
 }
 
-// every FFT shape of the tile kernels: powers of two, then 3 * 2^k (Bluestein only)
+// every FFT shape of the tile kernels: powers of two, then 3 * 2^k
 std::vector<int> all_shapes()
 {
     std::vector<int> v;
@@ -78,10 +78,11 @@ int mdft_tune_auto(double* seconds)
             if (tile_threads(shape, false) == tile_threads(shape, true)) continue;  // one tile size only
             const int m = mdft::m_of(shape);
             for (int blue = 0; blue < 2; ++blue) {
-                if ((blue && shape < 2) || (!blue && mdft::is_f3(shape))) continue;
-                // a length of this shape: m itself, m/2 + 1 for a power-of-two
-                // Bluestein convolution, m/2 for a 3 * 2^k one
-                const int n = !blue ? m : mdft::is_f3(shape) ? m / 2 : m / 2 + 1;
+                if (blue && shape < 2) continue;
+                // a length of this shape: m itself (direct), m/2 + 1 for a
+                // power-of-two Bluestein convolution, m/2 - 1 for a 3 * 2^k one
+                // (m/2 = 3 * 2^(k-1) may itself be a direct length)
+                const int n = !blue ? m : mdft::is_f3(shape) ? m / 2 - 1 : m / 2 + 1;
                 // 3 * 2^k convolutions are only chosen for columns; the tile
                 // kernels of a shape are tuned along both axes all the same
                 auto lp = ctx.line_plan(n, mdft::is_f3(shape));

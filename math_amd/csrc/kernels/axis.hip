@@ -49,14 +49,10 @@ static hipError_t launch_axis_tt(const AxisJob& job, bool blue, bool cols, hipSt
             return hipErrorInvalidValue;
         }
     } else {
-        if constexpr (!is_f3(S)) {
-            if (cols)
-                hipLaunchKernelGGL((k_axis<S, T, false, true>), dim3(grid), dim3(T), 0, stream, job);
-            else
-                hipLaunchKernelGGL((k_axis<S, T, false, false>), dim3(grid), dim3(T), 0, stream, job);
-        } else {
-            return hipErrorInvalidValue;
-        }
+        if (cols)
+            hipLaunchKernelGGL((k_axis<S, T, false, true>), dim3(grid), dim3(T), 0, stream, job);
+        else
+            hipLaunchKernelGGL((k_axis<S, T, false, false>), dim3(grid), dim3(T), 0, stream, job);
     }
     return hipGetLastError();
 }

@@ -75,12 +75,10 @@ __device__ __forceinline__ void run_tile_body(const AxisJob& job, int blue, int 
                 return;
             }
         }
-        if constexpr (!is_f3(S)) {
-            if (cols)
-                line_tile<S, T, false, true>(job, tile, lds, threadIdx.x);
-            else
-                line_tile<S, T, false, false>(job, tile, lds, threadIdx.x);
-        }
+        if (cols)
+            line_tile<S, T, false, true>(job, tile, lds, threadIdx.x);
+        else
+            line_tile<S, T, false, false>(job, tile, lds, threadIdx.x);
     }
 }
 

@@ -4,10 +4,10 @@
 // element stride es and line stride ls inside a row-major complex<float>
 // image (rows: es = 1, ls = w; columns: es = w, ls = 1).  A line of length n
 // is transformed through an FFT of length M: directly when n is a power of
-// two (M = n), otherwise by Bluestein's chirp-z method with M >= 2n - 1: the
-// reference's padded length bit_ceil(2n - 2) (src/gpu/dft/function.cpp:32-45),
-// or 3 * 2^k when that is shorter, and the chirp tables of function.cpp:49-92
-// generated in FP64 on the device.
+// two or 3 * 2^k in [192, 6144] (M = n), otherwise by Bluestein's chirp-z
+// method with M >= 2n - 1: the reference's padded length bit_ceil(2n - 2)
+// (src/gpu/dft/function.cpp:32-45), or 3 * 2^k when that is shorter, and the
+// chirp tables of function.cpp:49-92 generated in FP64 on the device.
 //
 // A workgroup owns LPW whole lines (a "tile").  Each thread holds EPT = 16
 // points of one line in VGPRs, at positions t + i*TPL (TPL = M/16 threads per
@@ -66,11 +66,12 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 // A line's FFT has a "shape" S:
 //  * S in [1, MAX_LOGM]: M = 2^S, radix-16 passes (16 points per thread) and
 //    one remainder pass;
-//  * S = F3_BASE + k: M = 3 * 2^k (Bluestein convolutions only), 12 points per
+//  * S = F3_BASE + k: M = 3 * 2^k (Bluestein convolutions, and lines of
+//    exactly 3 * 2^k points transformed directly), 12 points per
 //    thread: radix-4 passes, one radix-2 pass when k is odd, and a final
 //    radix-12 pass (a 3 x 4 prime-factor butterfly).
-// The shape is a function of the line length alone, so a line runs the same
-// schedule in every kernel.
+// The shape is a function of the line length and axis, so a line runs the
+// same schedule in every kernel.
 constexpr int MAX_LOGM = 14;
 // waves per SIMD the tile kernels are compiled for (at most 128 VGPRs)
 constexpr int TILE_WAVES = 4;
@@ -631,7 +632,6 @@ struct FftTile {
 template <int S, int THREADS, bool BLUE, bool COLS, int IO = 0>
 __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* lds, int tid, ImageIO io = ImageIO{})
 {
-    static_assert(BLUE || !is_f3(S), "3 * 2^k lengths are Bluestein convolutions");
     using C = LineCfg<S, THREADS>;
     const int ll = COLS ? (tid % C::LPW) : (tid / C::TPL);
     const int t = COLS ? (tid / C::LPW) : (tid % C::TPL);

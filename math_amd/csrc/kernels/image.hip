@@ -43,10 +43,7 @@ static hipError_t launch_image_tt(const AxisJob& job, const ImageIO& io, bool bl
         else
             return hipErrorInvalidValue;
     } else {
-        if constexpr (!is_f3(S))
-            hipLaunchKernelGGL((k_image_axis<S, T, false, COLS, IO>), dim3(grid), dim3(T), 0, stream, job, io);
-        else
-            return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_image_axis<S, T, false, COLS, IO>), dim3(grid), dim3(T), 0, stream, job, io);
     }
     return hipGetLastError();
 }

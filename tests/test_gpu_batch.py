@@ -101,3 +101,17 @@ def test_batch_rejects_oversize(env):
     M, _ = env
     with pytest.raises(M.ErrorException, match="batch padded-length limit"):
         M.Batch([(4099, 5)], [False], [0], [0])
+
+
+def test_batch_direct_three_times_power_of_two(env):
+    # lines of exactly 3 * 2^k points (192 .. 6144) run a direct FFT of that length
+    M, torch = env
+    rng = np.random.default_rng(33)
+    sizes = [(192, 768), (768, 192), (1536, 384), (384, 1536), (3072, 6), (6, 3072), (6144, 3), (3, 6144), (96, 192),
+             (193, 191)]
+    invs = [bool(i % 2) for i in range(len(sizes))]
+    xs = [rand_c64(rng, (h, w)) for (w, h) in sizes]
+    res = run_batch(M, torch, sizes, invs, xs, 0, False)[0]
+    for (w, h), inv, x, got in zip(sizes, invs, xs, res):
+        assert np.array_equal(got, single(M, x, inv)), f"{w}x{h}"
+        check_against_fp64(got, x, inv, f"batch 3*2^k {w}x{h}")

@@ -107,9 +107,10 @@ def test_all_line_lengths_1_to_300(mdft):
         check_against_fp64(got, xt, True, "cols")
 
 
-# 1025-1536 and 2049-3072 (and 2500) run 3 * 2^k convolutions along both axes, 3073 the next power of two
-@pytest.mark.parametrize("n", [511, 512, 513, 1000, 1023, 1024, 1025, 1536, 2047, 2048, 2049, 2500, 3072, 3073,
-                               4093, 4096, 4097, 8191, 8192, 8193])
+# 1025-1536 and 2049-3072 (and 2500) run 3 * 2^k convolutions along both axes, 3073 the next power of two;
+# 384, 768, 1536, 3072 and 6144 are 3 * 2^k themselves (direct FFTs)
+@pytest.mark.parametrize("n", [384, 511, 512, 513, 768, 1000, 1023, 1024, 1025, 1536, 2047, 2048, 2049, 2500, 3072,
+                               3073, 4093, 4096, 4097, 6144, 8191, 8192, 8193])
 def test_long_lines(mdft, n):
     rng = np.random.default_rng(n)
     x = rand_c64(rng, (2, n))

@@ -115,3 +115,34 @@ def test_batch_direct_three_times_power_of_two(env):
     for (w, h), inv, x, got in zip(sizes, invs, xs, res):
         assert np.array_equal(got, single(M, x, inv)), f"{w}x{h}"
         check_against_fp64(got, x, inv, f"batch 3*2^k {w}x{h}")
+
+
+def test_batch_phases_and_info(env):
+    # the two launch phases run separately (mdft_batch_exec_phase) give the
+    # same bits as the whole batch; mdft_batch_info accounts every item and
+    # the algorithmic bytes 16 * w * h per pass
+    M, torch = env
+    rng = np.random.default_rng(41)
+    sizes = mixed_sizes(rng, 40, 17, 1500) + [(1, 9), (9, 1), (24, 24)]
+    invs = [bool(rng.integers(0, 2)) for _ in sizes]
+    xs = [rand_c64(rng, (h, w)) for (w, h) in sizes]
+    whole = run_batch(M, torch, sizes, invs, xs, 1, False)[0]
+    offs, total = layout(sizes)
+    src = torch.from_numpy(np.concatenate([x.ravel() for x in xs])).cuda()
+    dst = torch.zeros_like(src)
+    b = M.Batch(sizes, invs, offs, offs, 1)
+    info = b.info()
+    s = torch.cuda.current_stream().cuda_stream
+    b.exec_phase(1, src.data_ptr(), dst.data_ptr(), s)
+    b.exec_phase(2, src.data_ptr(), dst.data_ptr(), s)
+    b.check()
+    torch.cuda.synchronize()
+    got = dst.cpu().numpy()
+    b.close()
+    for off, (w, h), ref in zip(offs, sizes, whole):
+        assert np.array_equal(got[off:off + w * h].reshape(h, w), ref), f"{w}x{h}: phases differ from the batch"
+    assert info["items"][0] > 0 and info["items"][1] > 0
+    assert info["threads"][0] in (256, 512, 1024) and info["threads"][1] >= info["threads"][0]
+    # every image with both passes moves 16*w*h bytes per pass; one-pass images 16*w*h in phase 1
+    expect = sum(16 * w * h * ((w > 1) + (h > 1)) for (w, h) in sizes)
+    assert sum(info["bytes"]) == expect

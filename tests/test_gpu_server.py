@@ -121,3 +121,29 @@ def test_replay_trace(env):
     st = s.stats()
     assert st["completed"] == n and st["in_flight"] == 0
     s.close()
+
+
+def test_batching_window(env):
+    # with a window (hold requests while another micro-batch runs until 8 wait
+    # or the oldest has waited 300 us) every result is still the alone result
+    M, torch = env
+    rng = np.random.default_rng(91)
+    sizes = log_uniform_sizes(rng, 80, 32, 400)
+    invs = [bool(rng.integers(0, 2)) for _ in sizes]
+    xs = [rand_c64(rng, (h, w)) for (w, h) in sizes]
+    srcs = [torch.from_numpy(x.ravel().copy()).cuda() for x in xs]
+    torch.cuda.synchronize()
+    s = M.Server(256 << 20, streams=2, max_batch=32)
+    s.set_batching(8, 300.0)
+    tickets = [s.submit(w, h, inv, src.data_ptr()) for (w, h), inv, src in zip(sizes, invs, srcs)]
+    for t, (w, h), inv, x in zip(tickets, sizes, invs, xs):
+        s.wait(t, 30_000_000)
+        got = fetch(s, t, w, h)
+        s.release(t)
+        assert np.array_equal(got, alone(M, torch, x, inv)), f"{w}x{h}: windowed result differs from alone"
+    st = s.stats()
+    s.close()
+    assert st["completed"] == len(sizes)
+    assert st["launches"] < len(sizes)
+    with pytest.raises(M.ErrorException):
+        M.Server(1 << 20, streams=1, max_batch=4).set_batching(2, -1.0)

@@ -95,6 +95,27 @@ def test_random_sweep_1_to_100(mdft):
         check_against_fp64(got, x, inv, "sweep")
 
 
+def test_random_sweep_to_3000(mdft):
+    # test_dft.cpp:437-449 draws 10 % of its sizes up to 1000; here log-uniform sides in [100, 3000]
+    # (every shape family on both axes: powers of two, 3 * 2^k and 2^k convolutions, direct 3 * 2^k),
+    # forward and inverse, through the device entry point
+    import torch
+
+    rng = np.random.default_rng(3000)
+    for _ in range(24):
+        w, h = (int(np.exp(rng.uniform(np.log(100), np.log(3001)))) for _ in range(2))
+        x = rand_c64(rng, (h, w))
+        inv = bool(rng.integers(0, 2))
+        src = torch.from_numpy(x.ravel().copy()).cuda()
+        dst = torch.empty_like(src)
+        p = mdft.Plan(w, h)
+        p.exec_device(inv, src.data_ptr(), dst.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        p.close()
+        log(f"sweep {w}x{h} inv={inv}")
+        check_against_fp64(dst.cpu().numpy().reshape(h, w), x, inv, "sweep to 3000")
+
+
 def test_all_line_lengths_1_to_300(mdft):
     # every line length of one axis: exercises every padded length up to 1024 and both paths
     rng = np.random.default_rng(7)

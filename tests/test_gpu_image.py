@@ -50,7 +50,9 @@ def check(got, pixels, x, y, w, h):
     return m
 
 
-@pytest.mark.parametrize("w,h", [(640, 480), (256, 256), (97, 61), (1, 37), (53, 1), (1, 1), (2, 3), (1024, 8)])
+# (192, 70): direct 3 * 2^k rows, 3 * 2^k convolution columns; (1100, 140): 3 * 2^k convolutions on both axes
+@pytest.mark.parametrize("w,h", [(640, 480), (256, 256), (97, 61), (1, 37), (53, 1), (1, 1), (2, 3), (1024, 8),
+                                 (192, 70), (1100, 140)])
 def test_rgba32f_rectangles(mdft, w, h):
     rng = np.random.default_rng(w * 131 + h)
     rows, cols = h + 9, w + 5

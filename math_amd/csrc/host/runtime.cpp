@@ -322,10 +322,10 @@ AxisJob Context::make_job(const LinePlan& lp, const float2* src, float2* dst, in
 }
 
 // Step of a long-line FFT: sub-FFTs of length 2^sublogm.
-static void large_step(int sublogm, const float2* tw, LargeJob job, hipStream_t stream)
+static void large_step(int sublogm, const float2* tw, LargeJob job, int cu_count, hipStream_t stream)
 {
     job.tw = tw;
-    hip_check(launch_large_step(sublogm, job, stream), "long-line step launch");
+    hip_check(launch_large_step(sublogm, job, cu_count, stream), "long-line step launch");
 }
 
 void Context::enqueue_large_fft(const LargeFft& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
@@ -342,7 +342,7 @@ void Context::enqueue_large_fft(const LargeFft& lp, const float2* src, float2* d
     a.load_mode = LARGE_LOAD_PLAIN, a.store_mode = LARGE_STORE_TWIDDLE;
     a.n = M, a.m1 = M1, a.m2 = M2, a.scale = 1.0f;
     a.tw_lo = lp.tw_lo, a.tw_hi = lp.tw_hi;
-    large_step(lp.logm2, twiddles(lp.logm2), a, stream);
+    large_step(lp.logm2, twiddles(lp.logm2), a, cu_count_, stream);
     LargeJob c{};
     c.src = scratch;
     c.dst = dst;
@@ -351,7 +351,7 @@ void Context::enqueue_large_fft(const LargeFft& lp, const float2* src, float2* d
     c.count = count * M2;
     c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = LARGE_STORE_CONJ;  // sign 0: a plain store times scale
     c.n = M, c.m1 = M1, c.m2 = M2, c.scale = scale;
-    large_step(lp.logm1, twiddles(lp.logm1), c, stream);
+    large_step(lp.logm1, twiddles(lp.logm1), c, cu_count_, stream);
 }
 
 void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
@@ -371,7 +371,7 @@ void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* 
         a.load_mode = load_mode, a.store_mode = LARGE_STORE_TWIDDLE, a.inverse = inverse ? 1 : 0;
         a.n = lp.n, a.m1 = M1, a.m2 = M2, a.scale = 1.0f;
         a.chirp = lp.chirp, a.tw_lo = lp.tw_lo, a.tw_hi = lp.tw_hi;
-        large_step(lp.logm2, twiddles(lp.logm2), a, stream);
+        large_step(lp.logm2, twiddles(lp.logm2), a, cu_count_, stream);
     };
     auto step_c = [&](float2* to, int64_t tls, int64_t tes, int store_mode, float sc) {
         LargeJob c{};
@@ -383,7 +383,7 @@ void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* 
         c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = store_mode, c.inverse = inverse ? 1 : 0;
         c.n = lp.n, c.m1 = M1, c.m2 = M2, c.scale = sc;
         c.chirp = lp.chirp, c.dtab = lp.dtab;
-        large_step(lp.logm1, twiddles(lp.logm1), c, stream);
+        large_step(lp.logm1, twiddles(lp.logm1), c, cu_count_, stream);
     };
     if (!lp.blue) {
         step_a(src, ls, es, LARGE_LOAD_CONJ);         // conj(x) for the inverse

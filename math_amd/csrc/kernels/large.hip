@@ -20,18 +20,16 @@
 
 namespace mdft {
 
-constexpr int LARGE_T = 1024;
-
 // line L of a step maps to (L / sub) * ls + (L % sub) * ls2; element p to p * es
 __device__ __forceinline__ int64_t line_base(int64_t L, int sub, int64_t ls, int64_t ls2)
 {
     return (L / sub) * ls + (L % sub) * ls2;
 }
 
-template <int LOGM>
-__global__ __launch_bounds__(LARGE_T) void k_large_step(LargeJob job)
+template <int LOGM, int T>
+__global__ __launch_bounds__(T) void k_large_step(LargeJob job)
 {
-    using C = LineCfg<LOGM, LARGE_T>;
+    using C = LineCfg<LOGM, T>;
     constexpr int LF = C::template lds_floats<true>();
     __shared__ float lds[LF > 0 ? LF : 1];
     const int tid = threadIdx.x;
@@ -65,7 +63,7 @@ __global__ __launch_bounds__(LARGE_T) void k_large_step(LargeJob job)
         }
     }
 
-    FftTile<LOGM, LARGE_T, true>::run(v, t, ll, lds, job.tw);
+    FftTile<LOGM, T, true>::run(v, t, ll, lds, job.tw);
 
     if (!valid) return;
     float2* o = job.dst + line_base(line, job.sub_out, job.ls_out, job.ls2_out);
@@ -93,34 +91,54 @@ __global__ __launch_bounds__(LARGE_T) void k_large_step(LargeJob job)
     }
 }
 
-template <int LOGM>
-static hipError_t launch_large_t(const LargeJob& job, hipStream_t stream)
+template <int LOGM, int T>
+static hipError_t launch_large_tt(const LargeJob& job, hipStream_t stream)
 {
-    constexpr int LPW = LineCfg<LOGM, LARGE_T>::LPW;
+    constexpr int LPW = LineCfg<LOGM, T>::LPW;
     const int64_t grid = (job.count + LPW - 1) / LPW;
     if (grid <= 0) return hipSuccess;
     if (grid > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_large_step<LOGM>, dim3((unsigned)grid), dim3(LARGE_T), 0, stream, job);
+    hipLaunchKernelGGL((k_large_step<LOGM, T>), dim3((unsigned)grid), dim3(T), 0, stream, job);
     return hipGetLastError();
 }
 
-hipError_t launch_large_step(int logm, const LargeJob& job, hipStream_t stream)
+// Tiles of 1024 threads (most sub-lines per tile, the widest contiguous runs
+// per strided row) unless that leaves fewer than min_blocks workgroups; then
+// the largest of 256 and 64 threads that does not, else the smallest tile
+// that holds a sub-line (a few long lines: every CU gets work).
+template <int LOGM>
+static hipError_t launch_large_t(const LargeJob& job, int min_blocks, hipStream_t stream)
+{
+    constexpr int TPL = m_of(LOGM) / ept_of(LOGM);
+    auto blocks = [&](int t) { return (job.count + t / TPL - 1) / (t / TPL); };
+    if constexpr (TPL <= 256) {
+        if (blocks(1024) < min_blocks) {
+            if constexpr (TPL <= 64) {
+                if (blocks(256) < min_blocks) return launch_large_tt<LOGM, 64>(job, stream);
+            }
+            return launch_large_tt<LOGM, 256>(job, stream);
+        }
+    }
+    return launch_large_tt<LOGM, 1024>(job, stream);
+}
+
+hipError_t launch_large_step(int logm, const LargeJob& job, int min_blocks, hipStream_t stream)
 {
     switch (logm) {
-    case 1: return launch_large_t<1>(job, stream);
-    case 2: return launch_large_t<2>(job, stream);
-    case 3: return launch_large_t<3>(job, stream);
-    case 4: return launch_large_t<4>(job, stream);
-    case 5: return launch_large_t<5>(job, stream);
-    case 6: return launch_large_t<6>(job, stream);
-    case 7: return launch_large_t<7>(job, stream);
-    case 8: return launch_large_t<8>(job, stream);
-    case 9: return launch_large_t<9>(job, stream);
-    case 10: return launch_large_t<10>(job, stream);
-    case 11: return launch_large_t<11>(job, stream);
-    case 12: return launch_large_t<12>(job, stream);
-    case 13: return launch_large_t<13>(job, stream);
-    case 14: return launch_large_t<14>(job, stream);
+    case 1: return launch_large_t<1>(job, min_blocks, stream);
+    case 2: return launch_large_t<2>(job, min_blocks, stream);
+    case 3: return launch_large_t<3>(job, min_blocks, stream);
+    case 4: return launch_large_t<4>(job, min_blocks, stream);
+    case 5: return launch_large_t<5>(job, min_blocks, stream);
+    case 6: return launch_large_t<6>(job, min_blocks, stream);
+    case 7: return launch_large_t<7>(job, min_blocks, stream);
+    case 8: return launch_large_t<8>(job, min_blocks, stream);
+    case 9: return launch_large_t<9>(job, min_blocks, stream);
+    case 10: return launch_large_t<10>(job, min_blocks, stream);
+    case 11: return launch_large_t<11>(job, min_blocks, stream);
+    case 12: return launch_large_t<12>(job, min_blocks, stream);
+    case 13: return launch_large_t<13>(job, min_blocks, stream);
+    case 14: return launch_large_t<14>(job, min_blocks, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -53,7 +53,8 @@ struct LargeJob {
     const float2* tw;  // Stockham rows of the sub-FFT length
 };
 
-hipError_t launch_large_step(int logm, const LargeJob& job, hipStream_t stream);
+// min_blocks: smaller tiles when 1024-thread ones would give fewer workgroups
+hipError_t launch_large_step(int logm, const LargeJob& job, int min_blocks, hipStream_t stream);
 hipError_t gen_large_twiddles(float2* lo, float2* hi, int64_t m, hipStream_t stream);
 hipError_t gen_h2_f32(float2* h2, const double2* chirp64, int64_t n, int64_t m, hipStream_t stream);
 

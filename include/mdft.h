@@ -133,10 +133,10 @@ int mdft_exec_device_noalloc(uint32_t width, uint32_t height, int inverse, const
 /* ---- tile-shape tuning ---------------------------------------------------- */
 
 /* Tile shape the single-image kernels use for lines whose FFT has the given
- * shape: 1..14 for 2^shape points, 38..43 for 3 * 2^(shape - 32) points (the
- * Bluestein convolution length used when 3 * 2^k >= 2n - 1 is shorter than
- * the reference's compute_m(n): along columns from 192 points, along rows
- * from 3072); Bluestein (blue != 0) or power-of-two, along
+ * shape: 1..14 for 2^shape points, 38..43 for 3 * 2^(shape - 32) points
+ * (lines of exactly 3 * 2^k points, and the Bluestein convolution length used
+ * when 3 * 2^k >= 2n - 1 is shorter than the reference's compute_m(n): along
+ * columns from 192 points, along rows from 3072); Bluestein (blue != 0) or power-of-two, along
  * rows (cols == 0) or columns: 0 = 256-thread tiles, 1 = 1024-thread tiles
  * (four times the lines per tile).  No counterpart in the reference, whose
  * tile size follows the device's shared-memory limit

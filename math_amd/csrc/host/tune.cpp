@@ -1,8 +1,9 @@
 // Autotuner of the single-image tile shapes: times the narrow (256-thread)
-// and wide (1024-thread) variant of every (padded length, Bluestein, axis)
-// kernel on the current device and keeps the faster in the context's table.
-// The table can be saved to and loaded from a text file, one line per entry:
-// "logm blue cols variant".
+// and wide (1024-thread) variant of every (FFT shape, Bluestein, axis) kernel
+// on the current device and keeps the faster in the context's table.  The
+// table can be saved to and loaded from a text file, one line per entry:
+// "shape blue cols variant".  The tile size only groups lines into
+// workgroups: a line's result is bitwise the same under either variant.
 #include "capi_util.hpp"
 
 #include <chrono>

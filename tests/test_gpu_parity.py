@@ -95,15 +95,15 @@ def test_random_sweep_1_to_100(mdft):
         check_against_fp64(got, x, inv, "sweep")
 
 
-def test_random_sweep_to_3000(mdft):
-    # test_dft.cpp:437-449 draws 10 % of its sizes up to 1000; here log-uniform sides in [100, 3000]
+def test_random_sweep_to_4096(mdft):
+    # test_dft.cpp:437-449 draws 10 % of its sizes up to 1000; here log-uniform sides in [100, 4096]
     # (every shape family on both axes: powers of two, 3 * 2^k and 2^k convolutions, direct 3 * 2^k),
     # forward and inverse, through the device entry point
     import torch
 
     rng = np.random.default_rng(3000)
-    for _ in range(24):
-        w, h = (int(np.exp(rng.uniform(np.log(100), np.log(3001)))) for _ in range(2))
+    for _ in range(160):
+        w, h = (int(np.exp(rng.uniform(np.log(100), np.log(4097)))) for _ in range(2))
         x = rand_c64(rng, (h, w))
         inv = bool(rng.integers(0, 2))
         src = torch.from_numpy(x.ravel().copy()).cuda()
@@ -113,7 +113,7 @@ def test_random_sweep_to_3000(mdft):
         torch.cuda.synchronize()
         p.close()
         log(f"sweep {w}x{h} inv={inv}")
-        check_against_fp64(dst.cpu().numpy().reshape(h, w), x, inv, "sweep to 3000")
+        check_against_fp64(dst.cpu().numpy().reshape(h, w), x, inv, "sweep to 4096")
 
 
 def test_all_line_lengths_1_to_300(mdft):

@@ -49,11 +49,11 @@ inline int shape_index(int shape) { return is_f3(shape) ? MAX_LOGM + 1 + (f3_k(s
 // Device tables of one line length n.
 struct LinePlan {
     int n = 0;
-    int m = 1;         // FFT length: n for a power of two, the Bluestein convolution length otherwise
+    int m = 1;         // FFT length: n for a direct line (2^k or 3 * 2^k), the Bluestein convolution length otherwise
     int logm = 0;      // log2(m) for power-of-two m (lines longer than one tile: the four-step split)
     int shape = 0;     // FFT shape of the tile kernels (line_shape(n, axis))
     int padded = 1;    // the reference's padded length compute_m(n) (reported by mdft_plan_info)
-    bool blue = false;  // Bluestein (n not a power of two)
+    bool blue = false;  // Bluestein (n neither 2^k nor a direct 3 * 2^k)
     float2* chirp = nullptr;  // n entries, Bluestein only
     float2* dtab = nullptr;   // m entries, Bluestein only
     // tile-resident Bluestein lengths: the direction tables (dir_tables_len)

@@ -645,7 +645,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
     // direction enters as a sign mask: conjugation is one xor, and the
     // inverse's 1/n is a multiply by job.scale (exactly 1 for the forward)
     const uint32_t sgn = job.inverse ? 0x80000000u : 0u;
-    // a power-of-two line fills the FFT (n == M); only Bluestein pads
+    // a direct line (2^k or 3 * 2^k points) fills the FFT (n == M); only Bluestein pads
     auto in_line = [&](int pos) { return !BLUE || pos < n; };
     // Bluestein: n <= M/2 + 1, so slot i holds a position of the line only
     // for i <= EPT/2 (slot EPT/2 only at t = 0 when n = M/2 + 1; for 3 * 2^k
@@ -667,7 +667,7 @@ __device__ __forceinline__ void line_tile(const AxisJob& job, int tile, float* l
                     x = make_float2(image_luminance(io, int64_t(line) * io.ils + int64_t(pos) * io.iel), 0.0f);
                 else
                     x = s[i * step];
-                // forward x*c, inverse x*conj(c) (direction tables); power of two: conj for the inverse
+                // forward x*c, inverse x*conj(c) (direction tables); direct lines: conj for the inverse
                 x = BLUE ? cmul(x, job.chirp[pos]) : cxor(x, sgn);
             }
             v[i] = x;

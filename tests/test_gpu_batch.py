@@ -146,3 +146,43 @@ def test_batch_phases_and_info(env):
     # every image with both passes moves 16*w*h bytes per pass; one-pass images 16*w*h in phase 1
     expect = sum(16 * w * h * ((w > 1) + (h > 1)) for (w, h) in sizes)
     assert sum(info["bytes"]) == expect
+
+
+@pytest.mark.parametrize("flags", [0, 1])
+def test_batch_empty_and_copy_only(env, flags):
+    # no transforms, and transforms of 1x1 images only (identities: a copy each)
+    M, torch = env
+    s = torch.cuda.current_stream().cuda_stream
+    buf = torch.zeros(4, dtype=torch.complex64, device="cuda")
+    b = M.Batch([], [], [], [], flags)
+    b.exec_device(buf.data_ptr(), buf.data_ptr(), s)
+    b.check()
+    assert b.info()["items"] == (0, 0)
+    b.close()
+    src = torch.tensor([1 + 2j, 3 - 4j, -5 + 0.5j], dtype=torch.complex64, device="cuda")
+    dst = torch.zeros_like(src)
+    b = M.Batch([(1, 1)] * 3, [False, True, False], [0, 1, 2], [2, 0, 1], flags)
+    b.exec_device(src.data_ptr(), dst.data_ptr(), s)
+    b.check()
+    torch.cuda.synchronize()
+    b.close()
+    assert dst.cpu().tolist() == [3 - 4j, -5 + 0.5j, 1 + 2j]
+
+
+def test_batch_phase_errors(env):
+    M, torch = env
+    s = torch.cuda.current_stream().cuda_stream
+    a = torch.zeros(64 * 64, dtype=torch.complex64, device="cuda")
+    c = torch.zeros_like(a)
+    b = M.Batch([(64, 64)], [False], [0], [0], M.MDFT_BATCH_PERSISTENT)
+    with pytest.raises(M.ErrorException, match="only grid-shape batches run by phase"):
+        b.exec_phase(1, a.data_ptr(), a.data_ptr(), s)
+    b.close()
+    b = M.Batch([(64, 64)], [False], [0], [0], 0)
+    with pytest.raises(M.ErrorException, match="phase must be 1 or 2"):
+        b.exec_phase(3, a.data_ptr(), a.data_ptr(), s)
+    b.exec_phase(1, a.data_ptr(), a.data_ptr(), s)
+    with pytest.raises(M.ErrorException, match="needs the pointers of the phase 1 call"):
+        b.exec_phase(2, c.data_ptr(), c.data_ptr(), s)
+    torch.cuda.synchronize()
+    b.close()

@@ -49,3 +49,20 @@ def test_autotune_save_load(mdft, tmp_path):
     check_against_fp64(got, x, False, "tuned")
     for k, v in before.items():
         mdft.tune_set(*k, v)
+
+
+def test_tune_errors(mdft, tmp_path):
+    before = table(mdft)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("# header\n10 0 1 1\n10 0 1 7\n")
+    with pytest.raises(mdft.ErrorException, match="bad entry on line 3"):
+        mdft.tune_load(str(bad))
+    assert table(mdft) == before, "a rejected file must change nothing"
+    with pytest.raises(mdft.ErrorException, match="cannot read"):
+        mdft.tune_load(str(tmp_path / "missing.txt"))
+    with pytest.raises(mdft.ErrorException, match="shape must be in"):
+        mdft.tune_get(15, 0, 0)
+    with pytest.raises(mdft.ErrorException, match="shape must be in"):
+        mdft.tune_set(37, 1, 0, 0)
+    with pytest.raises(mdft.ErrorException, match="variant must be 0 or 1"):
+        mdft.tune_set(10, 0, 0, 2)

@@ -359,8 +359,7 @@ void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* 
 {
     const int64_t M = lp.m, M1 = int64_t(1) << lp.logm1, M2 = int64_t(1) << lp.logm2;
     const float scale = inverse ? 1.0f / (float)lp.n : 1.0f;
-    float2* S = scratch;                      // step A output, count * M
-    float2* P = scratch + (size_t)count * M;  // Bluestein padded buffer, count * M
+    float2* S = scratch;  // step A output, count * M (later steps in place or out of it)
     auto step_a = [&](const float2* from, int64_t fls, int64_t fes, int load_mode) {
         LargeJob a{};
         a.src = from;
@@ -381,7 +380,7 @@ void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* 
         c.sub_out = M2, c.ls_out = tls, c.ls2_out = tes, c.es_out = M2 * tes;
         c.count = count * M2;
         c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = store_mode, c.inverse = inverse ? 1 : 0;
-        c.n = lp.n, c.m1 = M1, c.m2 = M2, c.scale = sc;
+        c.n = lp.n, c.m1 = M1, c.m2 = M2, c.kmul = M2, c.scale = sc;
         c.chirp = lp.chirp, c.dtab = lp.dtab;
         large_step(lp.logm1, twiddles(lp.logm1), c, cu_count_, stream);
     };
@@ -390,10 +389,31 @@ void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* 
         step_c(dst, ls, es, LARGE_STORE_CONJ, scale);  // conj(v)/n for the inverse
         return;
     }
-    step_a(src, ls, es, LARGE_LOAD_CHIRP);    // x * c, zero past n
-    step_c(P, M, 1, LARGE_STORE_DIAG, 1.0f);  // conj(v * D), natural order, padded
-    step_a(P, M, 1, LARGE_LOAD_PLAIN);
-    step_c(dst, ls, es, LARGE_STORE_CHIRP, scale);  // conj(v) * c, crop to n, 1/n for the inverse
+    step_a(src, ls, es, LARGE_LOAD_CHIRP);  // x * c, zero past n
+    // B, in place: per (line, k2) the M1-point FFT over n1, conj(v * D), the
+    // M1-point FFT over k1 and the twiddle w_M^(k2*j1), at j1*M2 + k2
+    LargeJob b{};
+    b.src = S;
+    b.dst = S;
+    b.sub_in = M2, b.ls_in = M, b.ls2_in = 1, b.es_in = M2;
+    b.sub_out = M2, b.ls_out = M, b.ls2_out = 1, b.es_out = M2;
+    b.count = count * M2;
+    b.load_mode = LARGE_LOAD_PLAIN, b.store_mode = LARGE_STORE_DIAG_FFT, b.inverse = inverse ? 1 : 0;
+    b.n = lp.n, b.m1 = M1, b.m2 = M2, b.kmul = M2, b.scale = 1.0f;
+    b.dtab = lp.dtab, b.tw_lo = lp.tw_lo, b.tw_hi = lp.tw_hi;
+    large_step(lp.logm1, twiddles(lp.logm1), b, cu_count_, stream);
+    // C', per (line, j1): the M2-point FFT over k2 into j = j1 + M1*j2:
+    // conj(v) * c, crop to n, 1/n for the inverse
+    LargeJob c{};
+    c.src = S;
+    c.dst = dst;
+    c.sub_in = M1, c.ls_in = M, c.ls2_in = M2, c.es_in = 1;
+    c.sub_out = M1, c.ls_out = ls, c.ls2_out = es, c.es_out = M1 * es;
+    c.count = count * M1;
+    c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = LARGE_STORE_CHIRP, c.inverse = inverse ? 1 : 0;
+    c.n = lp.n, c.m1 = M1, c.m2 = M2, c.kmul = M1, c.scale = scale;
+    c.chirp = lp.chirp;
+    large_step(lp.logm2, twiddles(lp.logm2), c, cu_count_, stream);
 }
 
 void Context::enqueue_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls, int count,

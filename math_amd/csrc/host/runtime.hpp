@@ -66,7 +66,7 @@ struct LinePlan {
     uint64_t bytes = 0;
     bool large() const { return logm > MAX_LOGM; }
     // scratch (complex<float> elements) one axis pass over `count` lines needs
-    uint64_t scratch_elems(int count) const { return large() ? (uint64_t)count * m * (blue ? 2 : 1) : 0; }
+    uint64_t scratch_elems(int count) const { return large() ? (uint64_t)count * m : 0; }
     ~LinePlan();
 };
 

@@ -8,10 +8,15 @@
 //           stored contiguously per (line, n1) in scratch;
 //   step C: for every (line, k2): length-M1 FFT over n1, stored at k2 + M2*k1
 //           (natural order).
-// A power-of-two line is A then C.  A Bluestein line is A (chirp and zero pad
-// on the load), C (times D, conjugated, into a padded buffer), A, C (conjugate,
-// chirp, crop, 1/n on the store).  Every step is a strided tile of whole
-// sub-lines run with the column (lines-fastest) mapping.
+// A power-of-two line is A then C.  A Bluestein line is three steps: A (chirp
+// and zero pad on the load); B, in place on the scratch, per k2: the length-M1
+// FFT over n1 (C's), times D and conjugated, then the first half of the
+// second FFT with the roles of M1 and M2 exchanged (k = k2 + M2*k1 and
+// j = j1 + M1*j2: w_M^(k*j) = w_M2^(k2*j2) * w_M^(k2*j1) * w_M1^(k1*j1)), i.e. a
+// length-M1 FFT over k1 times w_M^(k2*j1); and C', per j1: the length-M2 FFT
+// over k2 into j1 + M1*j2 (conjugate, chirp, crop, 1/n on the store).  Every
+// step is a strided tile of whole sub-lines run with the column
+// (lines-fastest) mapping.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -26,7 +31,7 @@ __device__ __forceinline__ int64_t line_base(int64_t L, int sub, int64_t ls, int
     return (L / sub) * ls + (L % sub) * ls2;
 }
 
-template <int LOGM, int T>
+template <int LOGM, int T, bool FUSED>
 __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
 {
     using C = LineCfg<LOGM, T>;
@@ -64,24 +69,34 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
     }
 
     FftTile<LOGM, T, true>::run(v, t, ll, lds, job.tw);
+    const int64_t q_line = line % job.sub_out;
+    if constexpr (FUSED) {
+        // conj(v * D) at k = k2 + M2*k1 (q_line = k2 < M2, pos = k1 < M1: in bounds on every line),
+        // then the next length-M1 FFT over k1; its outputs j1 take the twiddle below
+#pragma unroll
+        for (int i = 0; i < C::EPT; ++i) {
+            const int64_t k = q_line + job.kmul * (t + i * C::TPL);
+            v[i] = cmul_conj(v[i], cxor(job.dtab[k], sgn));
+        }
+        FftTile<LOGM, T, true>::run(v, t, ll, lds, job.tw);
+    }
 
     if (!valid) return;
     float2* o = job.dst + line_base(line, job.sub_out, job.ls_out, job.ls2_out);
-    const int64_t q_line = line % job.sub_out;
 #pragma unroll
     for (int i = 0; i < C::EPT; ++i) {
         const int64_t pos = t + i * C::TPL;
         float2 y = v[i];
-        if (job.store_mode == LARGE_STORE_TWIDDLE) {
+        if (job.store_mode == LARGE_STORE_TWIDDLE || job.store_mode == LARGE_STORE_DIAG_FFT) {
             // times w_M^(n1*k2), n1*k2 < M1*M2 = M: two-level table
             const int64_t q = q_line * pos;
             y = cmul(y, cmul(job.tw_lo[q & (LARGE_LO - 1)], job.tw_hi[q >> LARGE_LO_BITS]));
         } else if (job.store_mode == LARGE_STORE_DIAG) {
             // natural index k = k2 + M2*k1; forward conj(v*D), inverse conj(v*conj(D))
-            const int64_t k = q_line + job.m2 * pos;
+            const int64_t k = q_line + job.kmul * pos;
             y = cmul_conj(y, cxor(job.dtab[k], sgn));
         } else if (job.store_mode == LARGE_STORE_CHIRP) {
-            const int64_t k = q_line + job.m2 * pos;
+            const int64_t k = q_line + job.kmul * pos;
             if (k >= job.n) continue;
             y = cscale(cmul_conj(y, cxor(job.chirp[k], sgn ^ 0x80000000u)), job.scale);
         } else if (job.store_mode == LARGE_STORE_CONJ) {
@@ -98,7 +113,10 @@ static hipError_t launch_large_tt(const LargeJob& job, hipStream_t stream)
     const int64_t grid = (job.count + LPW - 1) / LPW;
     if (grid <= 0) return hipSuccess;
     if (grid > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_large_step<LOGM, T>), dim3((unsigned)grid), dim3(T), 0, stream, job);
+    if (job.store_mode == LARGE_STORE_DIAG_FFT)
+        hipLaunchKernelGGL((k_large_step<LOGM, T, true>), dim3((unsigned)grid), dim3(T), 0, stream, job);
+    else
+        hipLaunchKernelGGL((k_large_step<LOGM, T, false>), dim3((unsigned)grid), dim3(T), 0, stream, job);
     return hipGetLastError();
 }
 

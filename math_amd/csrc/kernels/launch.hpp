@@ -31,8 +31,11 @@ constexpr int LARGE_LO_BITS = 14;
 constexpr int64_t LARGE_LO = int64_t(1) << LARGE_LO_BITS;
 constexpr int LARGE_MAX_LOGM = 2 * MAX_LOGM;  // M1, M2 <= 2^14
 enum LargeLoad { LARGE_LOAD_PLAIN = 0, LARGE_LOAD_CONJ = 1, LARGE_LOAD_CHIRP = 2 };
+// LARGE_STORE_DIAG_FFT: conj(v * D) (natural index k2 + kmul*k1), a second FFT
+// of the same length, then the twiddle of LARGE_STORE_TWIDDLE (the middle of a
+// long Bluestein line in one step)
 enum LargeStore { LARGE_STORE_PLAIN = 0, LARGE_STORE_TWIDDLE = 1, LARGE_STORE_DIAG = 2, LARGE_STORE_CHIRP = 3,
-                  LARGE_STORE_CONJ = 4 };
+                  LARGE_STORE_CONJ = 4, LARGE_STORE_DIAG_FFT = 5 };
 
 // One step of a long-line FFT: count sub-lines; sub-line L reads from
 // src + (L / sub_in) * ls_in + (L % sub_in) * ls2_in with element stride es_in
@@ -45,6 +48,7 @@ struct LargeJob {
     int64_t count;
     int load_mode, store_mode, inverse, pad;
     int64_t n, m1, m2;
+    int64_t kmul;  // natural index of the store modes DIAG, DIAG_FFT, CHIRP: q + kmul * pos
     float scale;
     const float2* chirp;
     const float2* dtab;

@@ -33,7 +33,7 @@ def plan_result(M, torch, plan, inverse, src):
 def test_noalloc_matches_exec_device_two_calls(env):
     M, torch = env
     rng = np.random.default_rng(4242)
-    # long rows (padded 65536, Bluestein: 3 x 65536 x 2 scratch elements), then
+    # long rows (padded 65536, Bluestein: 3 x 65536 scratch elements), then
     # a smaller long-column case (padded 32768, power of two), then a tile-resident one
     cases = [(20000, 3, False), (2, 16385, True), (640, 480, True)]
     plans = {(w, h): M.Plan(w, h) for (w, h, _) in cases}  # keep the tables resident
@@ -69,9 +69,9 @@ def test_noalloc_in_place(env):
 def test_noalloc_errors(env):
     M, torch = env
     buf = torch.zeros(16, dtype=torch.complex64, device="cuda")
-    # 40000 x 40: the row pass needs 40 x 65536 x 2 scratch elements, above 4 Mi
+    # 40000 x 80: the row pass needs 80 x 65536 scratch elements, above 4 Mi
     with pytest.raises(M.ErrorException, match="above the static limit") as e:
-        M.exec_device_noalloc(40000, 40, False, buf.data_ptr(), buf.data_ptr())
+        M.exec_device_noalloc(40000, 80, False, buf.data_ptr(), buf.data_ptr())
     assert e.value.code == M.MDFT_ERR_SIZE
     with pytest.raises(M.ErrorException, match="for compute m"):
         M.exec_device_noalloc(0, 5, False, buf.data_ptr(), buf.data_ptr())

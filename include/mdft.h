@@ -60,10 +60,9 @@ typedef struct mdft_plan_s* mdft_plan;
  * diagonal tables resident on the device.  Replaces
  * ComputeVector::create_buffers(width, height) (src/gpu/dft/compute.h:56,
  * compute.cpp:229-255) and Dft::create_buffers (dft.h:38, dft.cpp:262-309).
+ * Every side the reference accepts runs (padded lengths up to 2^30).
  * Errors: MDFT_ERR_SIZE with "Error size <n> for compute m" when a side is
- * outside [1, 2^29], and "... exceeds the longest supported line (2^28)" when
- * its padded length is above 2^28 (a side of 2^29, or a non-power-of-two side
- * above 2^27 + 1). */
+ * outside [1, 2^29]. */
 int mdft_plan_create(mdft_plan* plan, uint32_t width, uint32_t height);
 
 /* Free a plan and the device memory it owns (its image copy and long-line
@@ -170,7 +169,8 @@ typedef struct mdft_fft_s* mdft_fft;
  * create_fft(device, family_indices, count, n) (src/gpu/dft/fft.h:49-53,
  * fft.cpp:293-330).  Errors (MDFT_ERR_SIZE, reference texts of
  * fft.cpp:298-306): "FFT size <n> is not positive", "FFT size <n> is not an
- * integral power of 2"; n above 2^28 or count*n above 2^40 are rejected. */
+ * integral power of 2"; n above 2^29 ("... exceeds the longest supported
+ * line (2^29)") or count*n above 2^40 are rejected. */
 int mdft_fft_create(mdft_fft* fft, uint32_t count, uint32_t n);
 
 /* In-place FFT of size complex<float> values in host memory, blocking.

@@ -32,9 +32,9 @@ int mdft_fft_create(mdft_fft* fft, uint32_t count, uint32_t n)
     if (n == 0) throw Error(MDFT_ERR_SIZE, "FFT size " + std::to_string(n) + " is not positive");
     if (!mdft::is_pow2(n))
         throw Error(MDFT_ERR_SIZE, "FFT size " + std::to_string(n) + " is not an integral power of 2");
-    if (n > (1u << mdft::LARGE_MAX_LOGM))
-        throw Error(MDFT_ERR_SIZE, "FFT size " + std::to_string(n) + " exceeds the longest supported line (2^" +
-                                       std::to_string(mdft::LARGE_MAX_LOGM) + ")");
+    // the longest line of a 2-D plan (compute_m's limit, function.cpp:34-38)
+    if (n > (1u << 29))
+        throw Error(MDFT_ERR_SIZE, "FFT size " + std::to_string(n) + " exceeds the longest supported line (2^29)");
     if (count > 0x7fffffffu || (uint64_t)count * n > (uint64_t(1) << 40))
         throw Error(MDFT_ERR_SIZE, "FFT count " + std::to_string(count) + " x size " + std::to_string(n) +
                                        " is too large");

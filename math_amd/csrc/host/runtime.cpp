@@ -5,6 +5,7 @@
 
 #include "../../../include/mdft.h"
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -35,6 +36,36 @@ int compute_m(long long n)
     uint64_t m = 1;
     while (m < (uint64_t)(2 * n - 2)) m <<= 1;
     return (int)m;
+}
+
+int three_factor_min_logm()
+{
+    static const int v = [] {
+        const char* e = std::getenv("MDFT_LARGE_SPLIT3_LOGM");
+        const int x = e ? std::atoi(e) : 0;
+        return x >= 15 && x <= 29 ? x : 29;
+    }();
+    return v;
+}
+
+void large_split(int logm, int* l1, int* l2, int* l3)
+{
+    if (logm >= three_factor_min_logm()) {
+        *l3 = logm / 3;
+        *l2 = (logm - *l3) / 2;
+        *l1 = logm - *l3 - *l2;
+    } else {
+        *l1 = logm - logm / 2;
+        *l2 = logm / 2;
+        *l3 = 0;
+    }
+}
+
+uint64_t large_scratch_elems(long long n, long long m, bool blue, long long count)
+{
+    if (m <= LineMaxTile) return 0;
+    const uint64_t per = (uint64_t)count * (uint64_t)m;
+    return blue && ilog2((uint64_t)m) >= three_factor_min_logm() ? 2 * per : per;
 }
 
 int line_shape(long long n, bool cols)
@@ -180,8 +211,7 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n, bool cols)
                                        ")");
     DeviceGuard g(device_);
     if (lp->large()) {
-        lp->logm1 = lp->logm - lp->logm / 2;
-        lp->logm2 = lp->logm / 2;
+        large_split(lp->logm, &lp->logm1, &lp->logm2, &lp->logm3);
         const int64_t nhi = int64_t(m) >> LARGE_LO_BITS;
         hip_check(hipMalloc(&lp->tw_lo, sizeof(float2) * LARGE_LO), "hipMalloc twiddles");
         hip_check(hipMalloc(&lp->tw_hi, sizeof(float2) * nhi), "hipMalloc twiddles");
@@ -189,10 +219,20 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n, bool cols)
         hip_check(gen_large_twiddles(lp->tw_lo, lp->tw_hi, m, gen_stream_), "twiddle generation");
     }
     if (lp->blue) {
+        struct DevFree {
+            void operator()(void* p) const { (void)hipFree(p); }
+        };
         // the FP64 scratch is sized at context creation for every tile-resident
-        // length; only long lines grow it (hipFree waits for the device)
+        // length; a long line's FP64 chirp (2n doubles) lives only while its
+        // tables are generated
+        std::unique_ptr<double, DevFree> chirp64_long;
+        if (lp->large()) {
+            double* raw64 = nullptr;
+            hip_check(hipMalloc(&raw64, sizeof(double) * 2 * (size_t)n), "hipMalloc FP64 chirp");
+            chirp64_long.reset(raw64);
+        }
         const size_t need = 2 * (size_t)n + (size_t)lp->m;
-        if (need > scratch64_doubles_) {
+        if (!lp->large() && need > scratch64_doubles_) {
             hip_check(hipStreamSynchronize(gen_stream_), "table generation");
             if (scratch64_) hip_check(hipFree(scratch64_), "hipFree");
             scratch64_ = nullptr;
@@ -204,7 +244,7 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n, bool cols)
         hip_check(hipMalloc(&lp->chirp, sizeof(float2) * (size_t)n), "hipMalloc chirp");
         hip_check(hipMalloc(&lp->dtab, sizeof(float2) * (size_t)cm), "hipMalloc dtab");
         lp->bytes += sizeof(float2) * ((size_t)n + (size_t)cm);
-        double2* chirp64 = reinterpret_cast<double2*>(scratch64_);
+        double2* chirp64 = reinterpret_cast<double2*>(lp->large() ? chirp64_long.get() : scratch64_);
         if (!lp->large()) {
             hip_check(hipMalloc(&lp->dir, sizeof(float2) * (size_t)dir_tables_len(n, cm)), "hipMalloc tables");
             lp->bytes += sizeof(float2) * (size_t)dir_tables_len(n, cm);
@@ -219,9 +259,6 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n, bool cols)
             // long lines: D = FFT_M(h2)/M by this library's four-step FFT on the
             // FP32-rounded embedding (the reference computes its D with an FP32 FFT
             // as well, dft.cpp:57-69)
-            struct DevFree {
-                void operator()(float2* p) const { (void)hipFree(p); }
-            };
             float2* raw = nullptr;
             hip_check(hipMalloc(&raw, sizeof(float2) * 2 * (size_t)m), "hipMalloc diagonal scratch");
             std::unique_ptr<float2, DevFree> tmp(raw);
@@ -230,6 +267,7 @@ std::shared_ptr<const LinePlan> Context::line_plan(int n, bool cols)
             f.m = m;
             f.logm1 = lp->logm1;
             f.logm2 = lp->logm2;
+            f.logm3 = lp->logm3;
             f.tw_lo = lp->tw_lo;
             f.tw_hi = lp->tw_hi;
             enqueue_large_fft(f, tmp.get(), lp->dtab, 1, m, 1, 1.0f / (float)m, tmp.get() + m, gen_stream_);
@@ -328,29 +366,87 @@ static void large_step(int sublogm, const float2* tw, LargeJob job, int cu_count
     hip_check(launch_large_step(sublogm, job, cu_count, stream), "long-line step launch");
 }
 
+// a step with no twiddle and two-level addressing
+static LargeJob large_job(const float2* tw_lo, const float2* tw_hi, bool inverse, int64_t n)
+{
+    LargeJob j{};
+    j.tmod = 1, j.tdiv = 1, j.tmul = 1;
+    j.scale = 1.0f;
+    j.inverse = inverse ? 1 : 0;
+    j.n = n;
+    j.tw_lo = tw_lo, j.tw_hi = tw_hi;
+    return j;
+}
+
+void Context::enqueue_large_fft3(const LargeFft& f, const float2* src, int64_t es, int64_t ls, float2* dst,
+                                 int64_t tes, int64_t tls, int64_t count, const LargeEnds& e, float2* T,
+                                 hipStream_t stream) const
+{
+    const int64_t M = f.m, M1 = int64_t(1) << f.logm1, M2 = int64_t(1) << f.logm2, M3 = int64_t(1) << f.logm3;
+    const int64_t Mr = M2 * M3;
+    // A1: per (line, j = n1 + M1*n2) the M3-point FFT over n3 (input stride
+    // M1*M2), times w_Mr^(n2*k3) = w_M^(M1*n2*k3), into T[line*M + j*M3 + k3]
+    LargeJob a = large_job(f.tw_lo, f.tw_hi, e.inverse, e.n);
+    a.src = src, a.dst = T;
+    a.sub_in = M1 * M2, a.ls_in = ls, a.ls2_in = es, a.es_in = M1 * M2 * es;
+    a.sub_out = M1 * M2, a.ls_out = M, a.ls2_out = M3, a.es_out = 1;
+    a.count = count * M1 * M2;
+    a.load_mode = e.load_mode, a.store_mode = LARGE_STORE_TWIDDLE;
+    a.tmod = M1 * M2, a.tdiv = M1, a.tmul = M1;
+    a.m1 = M1 * M2;  // the chirp load's element index: j + M1*M2*n3
+    a.chirp = e.chirp;
+    large_step(f.logm3, twiddles(f.logm3), a, cu_count_, stream);
+    // A2, in place: per (line, q = n1*M3 + k3) the M2-point FFT over n2 (stride M1*M3)
+    LargeJob b = large_job(f.tw_lo, f.tw_hi, e.inverse, e.n);
+    b.src = T, b.dst = T;
+    b.sub_in = M1 * M3, b.ls_in = M, b.ls2_in = 1, b.es_in = M1 * M3;
+    b.sub_out = M1 * M3, b.ls_out = M, b.ls2_out = 1, b.es_out = M1 * M3;
+    b.count = count * M1 * M3;
+    b.load_mode = LARGE_LOAD_PLAIN, b.store_mode = LARGE_STORE_PLAIN;
+    large_step(f.logm2, twiddles(f.logm2), b, cu_count_, stream);
+    // C: per (line, kr = k3 + M3*k2), element n1 at line*M + n1*M3 + k3 +
+    // k2*M1*M3, times w_M^(n1*kr) on the load, the M1-point FFT over n1, stored
+    // at kr + Mr*k1
+    LargeJob c = large_job(f.tw_lo, f.tw_hi, e.inverse, e.n);
+    c.src = T, c.dst = dst;
+    c.sub_in = Mr, c.ls_in = M, c.sub2_in = M3, c.ls2_in = 1, c.ls3_in = M1 * M3, c.es_in = M3;
+    c.sub_out = Mr, c.ls_out = tls, c.ls2_out = tes, c.es_out = Mr * tes;
+    c.count = count * Mr;
+    c.load_mode = LARGE_LOAD_TWIDDLE, c.tmod = Mr;
+    c.store_mode = e.store_mode, c.kmul = Mr, c.scale = e.scale;
+    c.chirp = e.chirp, c.dtab = e.dtab;
+    large_step(f.logm1, twiddles(f.logm1), c, cu_count_, stream);
+}
+
 void Context::enqueue_large_fft(const LargeFft& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
                                 int64_t count, float scale, float2* scratch, hipStream_t stream) const
 {
-    // forward FFT (power of two) of count lines: A then C
     const int64_t M = lp.m, M1 = int64_t(1) << lp.logm1, M2 = int64_t(1) << lp.logm2;
-    LargeJob a{};
+    if (lp.logm3 > 0) {
+        // sign 0: the store of LARGE_STORE_CONJ is a plain store times scale
+        const LargeEnds e{LARGE_LOAD_PLAIN, LARGE_STORE_CONJ, false, M, scale, nullptr, nullptr};
+        enqueue_large_fft3(lp, src, es, ls, dst, es, ls, count, e, scratch, stream);
+        return;
+    }
+    // forward FFT (power of two) of count lines: A then C
+    LargeJob a = large_job(lp.tw_lo, lp.tw_hi, false, M);
     a.src = src;
     a.dst = scratch;
     a.sub_in = M1, a.ls_in = ls, a.ls2_in = es, a.es_in = M1 * es;
     a.sub_out = M1, a.ls_out = M, a.ls2_out = M2, a.es_out = 1;
     a.count = count * M1;
     a.load_mode = LARGE_LOAD_PLAIN, a.store_mode = LARGE_STORE_TWIDDLE;
-    a.n = M, a.m1 = M1, a.m2 = M2, a.scale = 1.0f;
-    a.tw_lo = lp.tw_lo, a.tw_hi = lp.tw_hi;
+    a.tmod = M1;  // w_M^(n1*k2)
+    a.m1 = M1, a.m2 = M2;
     large_step(lp.logm2, twiddles(lp.logm2), a, cu_count_, stream);
-    LargeJob c{};
+    LargeJob c = large_job(lp.tw_lo, lp.tw_hi, false, M);
     c.src = scratch;
     c.dst = dst;
     c.sub_in = M2, c.ls_in = M, c.ls2_in = 1, c.es_in = M2;
     c.sub_out = M2, c.ls_out = ls, c.ls2_out = es, c.es_out = M2 * es;
     c.count = count * M2;
     c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = LARGE_STORE_CONJ;  // sign 0: a plain store times scale
-    c.n = M, c.m1 = M1, c.m2 = M2, c.scale = scale;
+    c.m1 = M1, c.m2 = M2, c.scale = scale;
     large_step(lp.logm1, twiddles(lp.logm1), c, cu_count_, stream);
 }
 
@@ -359,28 +455,46 @@ void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* 
 {
     const int64_t M = lp.m, M1 = int64_t(1) << lp.logm1, M2 = int64_t(1) << lp.logm2;
     const float scale = inverse ? 1.0f / (float)lp.n : 1.0f;
+    if (lp.logm3 > 0) {
+        const LargeFft f{M, lp.logm1, lp.logm2, lp.logm3, lp.tw_lo, lp.tw_hi};
+        if (!lp.blue) {
+            // conj(x) for the inverse on the load, conj(v)/n on the store
+            const LargeEnds e{LARGE_LOAD_CONJ, LARGE_STORE_CONJ, inverse, lp.n, scale, nullptr, nullptr};
+            enqueue_large_fft3(f, src, es, ls, dst, es, ls, count, e, scratch, stream);
+            return;
+        }
+        // the chirped line's FFT times D into U (natural order, conjugated),
+        // then the FFT of U with the chirp, crop and 1/n on the store
+        float2* U = scratch + count * M;
+        const LargeEnds e1{LARGE_LOAD_CHIRP, LARGE_STORE_DIAG, inverse, lp.n, 1.0f, lp.chirp, lp.dtab};
+        enqueue_large_fft3(f, src, es, ls, U, 1, M, count, e1, scratch, stream);
+        const LargeEnds e2{LARGE_LOAD_PLAIN, LARGE_STORE_CHIRP, inverse, lp.n, scale, lp.chirp, nullptr};
+        enqueue_large_fft3(f, U, 1, M, dst, es, ls, count, e2, scratch, stream);
+        return;
+    }
     float2* S = scratch;  // step A output, count * M (later steps in place or out of it)
     auto step_a = [&](const float2* from, int64_t fls, int64_t fes, int load_mode) {
-        LargeJob a{};
+        LargeJob a = large_job(lp.tw_lo, lp.tw_hi, inverse, lp.n);
         a.src = from;
         a.dst = S;
         a.sub_in = M1, a.ls_in = fls, a.ls2_in = fes, a.es_in = M1 * fes;
         a.sub_out = M1, a.ls_out = M, a.ls2_out = M2, a.es_out = 1;
         a.count = count * M1;
-        a.load_mode = load_mode, a.store_mode = LARGE_STORE_TWIDDLE, a.inverse = inverse ? 1 : 0;
-        a.n = lp.n, a.m1 = M1, a.m2 = M2, a.scale = 1.0f;
-        a.chirp = lp.chirp, a.tw_lo = lp.tw_lo, a.tw_hi = lp.tw_hi;
+        a.load_mode = load_mode, a.store_mode = LARGE_STORE_TWIDDLE;
+        a.tmod = M1;  // w_M^(n1*k2)
+        a.m1 = M1, a.m2 = M2;
+        a.chirp = lp.chirp;
         large_step(lp.logm2, twiddles(lp.logm2), a, cu_count_, stream);
     };
     auto step_c = [&](float2* to, int64_t tls, int64_t tes, int store_mode, float sc) {
-        LargeJob c{};
+        LargeJob c = large_job(lp.tw_lo, lp.tw_hi, inverse, lp.n);
         c.src = S;
         c.dst = to;
         c.sub_in = M2, c.ls_in = M, c.ls2_in = 1, c.es_in = M2;
         c.sub_out = M2, c.ls_out = tls, c.ls2_out = tes, c.es_out = M2 * tes;
         c.count = count * M2;
-        c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = store_mode, c.inverse = inverse ? 1 : 0;
-        c.n = lp.n, c.m1 = M1, c.m2 = M2, c.kmul = M2, c.scale = sc;
+        c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = store_mode;
+        c.m1 = M1, c.m2 = M2, c.kmul = M2, c.scale = sc;
         c.chirp = lp.chirp, c.dtab = lp.dtab;
         large_step(lp.logm1, twiddles(lp.logm1), c, cu_count_, stream);
     };
@@ -392,26 +506,27 @@ void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* 
     step_a(src, ls, es, LARGE_LOAD_CHIRP);  // x * c, zero past n
     // B, in place: per (line, k2) the M1-point FFT over n1, conj(v * D), the
     // M1-point FFT over k1 and the twiddle w_M^(k2*j1), at j1*M2 + k2
-    LargeJob b{};
+    LargeJob b = large_job(lp.tw_lo, lp.tw_hi, inverse, lp.n);
     b.src = S;
     b.dst = S;
     b.sub_in = M2, b.ls_in = M, b.ls2_in = 1, b.es_in = M2;
     b.sub_out = M2, b.ls_out = M, b.ls2_out = 1, b.es_out = M2;
     b.count = count * M2;
-    b.load_mode = LARGE_LOAD_PLAIN, b.store_mode = LARGE_STORE_DIAG_FFT, b.inverse = inverse ? 1 : 0;
-    b.n = lp.n, b.m1 = M1, b.m2 = M2, b.kmul = M2, b.scale = 1.0f;
-    b.dtab = lp.dtab, b.tw_lo = lp.tw_lo, b.tw_hi = lp.tw_hi;
+    b.load_mode = LARGE_LOAD_PLAIN, b.store_mode = LARGE_STORE_DIAG_FFT;
+    b.tmod = M2;  // w_M^(k2*j1)
+    b.m1 = M1, b.m2 = M2, b.kmul = M2;
+    b.dtab = lp.dtab;
     large_step(lp.logm1, twiddles(lp.logm1), b, cu_count_, stream);
     // C', per (line, j1): the M2-point FFT over k2 into j = j1 + M1*j2:
     // conj(v) * c, crop to n, 1/n for the inverse
-    LargeJob c{};
+    LargeJob c = large_job(lp.tw_lo, lp.tw_hi, inverse, lp.n);
     c.src = S;
     c.dst = dst;
     c.sub_in = M1, c.ls_in = M, c.ls2_in = M2, c.es_in = 1;
     c.sub_out = M1, c.ls_out = ls, c.ls2_out = es, c.es_out = M1 * es;
     c.count = count * M1;
-    c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = LARGE_STORE_CHIRP, c.inverse = inverse ? 1 : 0;
-    c.n = lp.n, c.m1 = M1, c.m2 = M2, c.kmul = M1, c.scale = scale;
+    c.load_mode = LARGE_LOAD_PLAIN, c.store_mode = LARGE_STORE_CHIRP;
+    c.m1 = M1, c.m2 = M2, c.kmul = M1, c.scale = scale;
     c.chirp = lp.chirp;
     large_step(lp.logm2, twiddles(lp.logm2), c, cu_count_, stream);
 }

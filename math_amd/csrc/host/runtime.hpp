@@ -46,6 +46,18 @@ int line_shape(long long n, bool cols);
 constexpr int NSHAPES = MAX_LOGM + 1 + (F3_MAX_K - F3_MIN_K + 1);
 inline int shape_index(int shape) { return is_f3(shape) ? MAX_LOGM + 1 + (f3_k(shape) - F3_MIN_K) : shape; }
 
+// Lines whose padded length is 2^three_factor_min_logm() or more split in
+// three factors (2^29 unless MDFT_LARGE_SPLIT3_LOGM in [15, 29] says otherwise,
+// read once per process: a testing knob that runs the three-factor steps on
+// lengths the oracle checks in full).
+int three_factor_min_logm();
+// the factors (log2) of a padded length 2^logm above one tile; l3 = 0: two
+void large_split(int logm, int* l1, int* l2, int* l3);
+// scratch of one axis pass over `count` lines of length n, padded length m:
+// count * m elements above one tile, twice that for a three-factor Bluestein
+// line (its product between the two FFTs), none otherwise
+uint64_t large_scratch_elems(long long n, long long m, bool blue, long long count);
+
 // Device tables of one line length n.
 struct LinePlan {
     int n = 0;
@@ -58,25 +70,37 @@ struct LinePlan {
     float2* dtab = nullptr;   // m entries, Bluestein only
     // tile-resident Bluestein lengths: the direction tables (dir_tables_len)
     float2* dir = nullptr;
-    // lines longer than one tile: M = 2^logm1 * 2^logm2 (four-step), with a
+    // lines longer than one tile: M = 2^logm1 * 2^logm2 (four-step), or
+    // 2^logm1 * 2^logm2 * 2^logm3 (logm3 > 0: three factors), with a
     // two-level table of exp(-2*pi*i*j/M)
-    int logm1 = 0, logm2 = 0;
+    int logm1 = 0, logm2 = 0, logm3 = 0;
     float2* tw_lo = nullptr;
     float2* tw_hi = nullptr;
     uint64_t bytes = 0;
     bool large() const { return logm > MAX_LOGM; }
     // scratch (complex<float> elements) one axis pass over `count` lines needs
-    uint64_t scratch_elems(int count) const { return large() ? (uint64_t)count * m : 0; }
+    uint64_t scratch_elems(int count) const { return large_scratch_elems(n, m, blue, count); }
     ~LinePlan();
 };
 
-// Geometry and twiddles of a power-of-two four-step FFT of length m (a view:
-// the tables belong to a LinePlan).
+// Geometry and twiddles of a power-of-two long FFT of length m (a view: the
+// tables belong to a LinePlan).
 struct LargeFft {
     int64_t m = 0;
-    int logm1 = 0, logm2 = 0;
+    int logm1 = 0, logm2 = 0, logm3 = 0;
     const float2* tw_lo = nullptr;
     const float2* tw_hi = nullptr;
+};
+
+// The first step's load and the last step's store of a three-factor FFT.
+struct LargeEnds {
+    int load_mode = 0;   // LargeLoad (plain, conj, chirp)
+    int store_mode = 0;  // LargeStore (conj with scale, diag, chirp)
+    bool inverse = false;
+    int64_t n = 0;       // line length (the chirp load's zero padding, the chirp store's crop)
+    float scale = 1.0f;
+    const float2* chirp = nullptr;
+    const float2* dtab = nullptr;
 };
 
 class Context {
@@ -114,6 +138,11 @@ public:
     // dst, both natural order), the building block of the long-line path.
     void enqueue_large_axis(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
                             int64_t count, bool inverse, float2* scratch, hipStream_t stream) const;
+    // Three-factor power-of-two FFT of length f.m (f.logm3 > 0) of count
+    // lines, src (element stride es, line stride ls) into dst (tes, tls),
+    // through T (count * m elements); e gives the ends' load and store.
+    void enqueue_large_fft3(const LargeFft& f, const float2* src, int64_t es, int64_t ls, float2* dst, int64_t tes,
+                            int64_t tls, int64_t count, const LargeEnds& e, float2* T, hipStream_t stream) const;
     // Forward power-of-two four-step FFT of length f.m, times scale
     // (scratch: count * m elements).
     void enqueue_large_fft(const LargeFft& f, const float2* src, float2* dst, int64_t es, int64_t ls, int64_t count,

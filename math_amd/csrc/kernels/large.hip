@@ -17,6 +17,16 @@
 // over k2 into j1 + M1*j2 (conjugate, chirp, crop, 1/n on the store).  Every
 // step is a strided tile of whole sub-lines run with the column
 // (lines-fastest) mapping.
+//
+// Padded lengths 2^29 and 2^30 (M2 * M3 above one tile) take three factors,
+// n = n1 + M1*(n2 + M2*n3), k = kr + Mr*k1 with Mr = M2*M3, kr = k3 + M3*k2:
+//   A1: per (line, n1, n2) the M3-point FFT over n3, times w_Mr^(n2*k3);
+//   A2: in place, per (line, n1, k3) the M2-point FFT over n2 (the Mr-point
+//       FFT of every n1, its output kr spread as k3 + M1*M3*k2 beside n1);
+//   C:  per (line, kr) times w_M^(n1*kr) on the load, the M1-point FFT over n1,
+//       stored at kr + Mr*k1.
+// A Bluestein line runs A1 (chirp), A2, C (times D), then A1, A2, C (chirp,
+// crop) on the product, through two scratch buffers.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -26,9 +36,24 @@
 namespace mdft {
 
 // line L of a step maps to (L / sub) * ls + (L % sub) * ls2; element p to p * es
-__device__ __forceinline__ int64_t line_base(int64_t L, int sub, int64_t ls, int64_t ls2)
+__device__ __forceinline__ int64_t line_base(int64_t L, int64_t sub, int64_t ls, int64_t ls2)
 {
     return (L / sub) * ls + (L % sub) * ls2;
+}
+
+// the input side, whose r = L % sub may split once more (sub2 > 0)
+__device__ __forceinline__ int64_t line_base_in(int64_t L, const LargeJob& job)
+{
+    const int64_t r = L % job.sub_in;
+    const int64_t off = job.sub2_in > 0 ? (r % job.sub2_in) * job.ls2_in + (r / job.sub2_in) * job.ls3_in
+                                        : r * job.ls2_in;
+    return (L / job.sub_in) * job.ls_in + off;
+}
+
+// w_M^e, e < M, from the two-level table
+__device__ __forceinline__ float2 large_twiddle(const LargeJob& job, int64_t e)
+{
+    return cmul(job.tw_lo[e & (LARGE_LO - 1)], job.tw_hi[e >> LARGE_LO_BITS]);
 }
 
 template <int LOGM, int T, bool FUSED>
@@ -45,10 +70,12 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
     const uint32_t sgn = job.inverse ? 0x80000000u : 0u;
     // sub-line coordinate inside the long line: n1 (step A) or k2 (step C)
     const int64_t sub_idx = line % job.sub_in;
+    // twiddle exponent per element of this sub-line
+    const int64_t tq = job.tmod > 0 && job.tdiv > 0 ? ((line % job.tmod) / job.tdiv) * job.tmul : 0;
 
     float2 v[C::EPT];
     {
-        const float2* s = job.src + (valid ? line_base(line, job.sub_in, job.ls_in, job.ls2_in) : 0);
+        const float2* s = job.src + (valid ? line_base_in(line, job) : 0);
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
             const int64_t pos = t + i * C::TPL;
@@ -60,6 +87,8 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
                     if (j < job.n) x = cmul(s[pos * job.es_in], cxor(job.chirp[j], sgn));
                 } else if (job.load_mode == LARGE_LOAD_CONJ) {
                     x = cxor(s[pos * job.es_in], sgn);
+                } else if (job.load_mode == LARGE_LOAD_TWIDDLE) {
+                    x = cmul(s[pos * job.es_in], large_twiddle(job, tq * pos));
                 } else {
                     x = s[pos * job.es_in];
                 }
@@ -88,9 +117,8 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
         const int64_t pos = t + i * C::TPL;
         float2 y = v[i];
         if (job.store_mode == LARGE_STORE_TWIDDLE || job.store_mode == LARGE_STORE_DIAG_FFT) {
-            // times w_M^(n1*k2), n1*k2 < M1*M2 = M: two-level table
-            const int64_t q = q_line * pos;
-            y = cmul(y, cmul(job.tw_lo[q & (LARGE_LO - 1)], job.tw_hi[q >> LARGE_LO_BITS]));
+            // times w_M^(n1*k2) (two factors: n1*k2 < M1*M2 = M), or w_Mr^(n2*k3) = w_M^(M1*n2*k3) (A1 of three)
+            y = cmul(y, large_twiddle(job, tq * pos));
         } else if (job.store_mode == LARGE_STORE_DIAG) {
             // natural index k = k2 + M2*k1; forward conj(v*D), inverse conj(v*conj(D))
             const int64_t k = q_line + job.kmul * pos;
@@ -112,7 +140,10 @@ static hipError_t launch_large_tt(const LargeJob& job, hipStream_t stream)
     constexpr int LPW = LineCfg<LOGM, T>::LPW;
     const int64_t grid = (job.count + LPW - 1) / LPW;
     if (grid <= 0) return hipSuccess;
-    if (grid > 0x7fffffff) return hipErrorInvalidValue;
+    if (grid > 0x7fffffff || job.sub_in < 1 || job.sub_out < 1 || job.sub2_in < 0) return hipErrorInvalidValue;
+    const bool twiddled = job.load_mode == LARGE_LOAD_TWIDDLE || job.store_mode == LARGE_STORE_TWIDDLE ||
+                          job.store_mode == LARGE_STORE_DIAG_FFT;
+    if (twiddled && (job.tmod < 1 || job.tdiv < 1 || job.tmul < 1)) return hipErrorInvalidValue;
     if (job.store_mode == LARGE_STORE_DIAG_FFT)
         hipLaunchKernelGGL((k_large_step<LOGM, T, true>), dim3((unsigned)grid), dim3(T), 0, stream, job);
     else

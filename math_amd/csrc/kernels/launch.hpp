@@ -29,8 +29,11 @@ constexpr uint32_t BATCH_WHOLE = 0x80000000u;
 // ---- lines longer than one tile (four-step through a scratch buffer) ----
 constexpr int LARGE_LO_BITS = 14;
 constexpr int64_t LARGE_LO = int64_t(1) << LARGE_LO_BITS;
-constexpr int LARGE_MAX_LOGM = 2 * MAX_LOGM;  // M1, M2 <= 2^14
-enum LargeLoad { LARGE_LOAD_PLAIN = 0, LARGE_LOAD_CONJ = 1, LARGE_LOAD_CHIRP = 2 };
+// padded lengths up to 2^28 split in two factors M1 * M2 (each <= 2^14), longer
+// ones (2^29 and 2^30, the reference's longest: compute_m of n <= 2^29) in three
+constexpr int LARGE_MAX_LOGM = 30;
+// LARGE_LOAD_TWIDDLE: the load multiplies by the twiddle of LARGE_STORE_TWIDDLE
+enum LargeLoad { LARGE_LOAD_PLAIN = 0, LARGE_LOAD_CONJ = 1, LARGE_LOAD_CHIRP = 2, LARGE_LOAD_TWIDDLE = 3 };
 // LARGE_STORE_DIAG_FFT: conj(v * D) (natural index k2 + kmul*k1), a second FFT
 // of the same length, then the twiddle of LARGE_STORE_TWIDDLE (the middle of a
 // long Bluestein line in one step)
@@ -38,13 +41,18 @@ enum LargeStore { LARGE_STORE_PLAIN = 0, LARGE_STORE_TWIDDLE = 1, LARGE_STORE_DI
                   LARGE_STORE_CONJ = 4, LARGE_STORE_DIAG_FFT = 5 };
 
 // One step of a long-line FFT: count sub-lines; sub-line L reads from
-// src + (L / sub_in) * ls_in + (L % sub_in) * ls2_in with element stride es_in
-// and writes likewise with the *_out fields.
+// src + (L / sub_in) * ls_in + r * ls2_in with r = L % sub_in and element
+// stride es_in (with sub2_in > 0: (r % sub2_in) * ls2_in + (r / sub2_in) *
+// ls3_in in place of r * ls2_in), and writes likewise with the *_out fields.
+// The twiddle of LARGE_LOAD_TWIDDLE, LARGE_STORE_TWIDDLE and
+// LARGE_STORE_DIAG_FFT is w_M^e of element pos, e = ((L % tmod) / tdiv) * tmul * pos.
 struct LargeJob {
     const float2* src;
     float2* dst;
     int64_t ls_in, ls2_in, es_in, sub_in;
+    int64_t sub2_in, ls3_in;
     int64_t ls_out, ls2_out, es_out, sub_out;
+    int64_t tmod, tdiv, tmul;
     int64_t count;
     int load_mode, store_mode, inverse, pad;
     int64_t n, m1, m2;

@@ -283,5 +283,9 @@ def test_lines_longer_than_one_tile(mdft, w, h):
 
 
 def test_longest_line_limit_message(mdft):
+    # the longest side is compute_m's (function.cpp:34-38); the lines up to it run in
+    # tests/test_gpu_long_lines.py
+    with pytest.raises(mdft.ErrorException, match="Error size 536870913 for compute m"):
+        mdft.Plan((1 << 29) + 1, 1)
     with pytest.raises(mdft.ErrorException, match="exceeds the longest supported line"):
-        mdft.Plan((1 << 27) + 3, 1)
+        mdft.create_fft(1, 1 << 30)

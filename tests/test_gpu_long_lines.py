@@ -122,3 +122,32 @@ def test_longest_lines(mdft, w, h):
     p.close()
     del x, X, y, x64
     torch.cuda.empty_cache()
+
+
+def test_fft1d_longest(mdft):
+    # the batched 1-D FFT at its longest length, two lines back to back
+    import torch
+
+    count, n = 2, 1 << 29
+    log(f"fft1d {count} x {n}")
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.complex(torch.rand((count, n), generator=g, device="cuda") * 2 - 1,
+                      torch.rand((count, n), generator=g, device="cuda") * 2 - 1)
+    f = mdft.Fft(count, n)
+    s = torch.cuda.current_stream().cuda_stream
+    X = torch.empty_like(x)
+    f.exec_device(False, x.data_ptr(), X.data_ptr(), s)
+    y = torch.empty_like(x)
+    f.exec_device(True, X.data_ptr(), y.data_ptr(), s)
+    torch.cuda.synchronize()
+    tol = 2e-6 * math.log2(n)
+    ks = [0, 1, n - 1, n // 2, 12345, 98765431]
+    for c in range(count):
+        norm = torch.linalg.vector_norm(x[c].to(torch.complex128)).item()
+        err = (X[c][ks].to(torch.complex128) - fp64_bins(x[c], ks, False)).abs().max().item() / norm
+        rt = (torch.linalg.vector_norm((y[c] - x[c]).to(torch.complex128)) / norm).item()
+        log(f"  line {c}: max bin error / RMS {err:.2e}, round trip {rt:.2e} (bound {tol:.2e})")
+        assert err <= tol and rt <= 2 * tol
+    f.close()
+    del x, X, y
+    torch.cuda.empty_cache()

@@ -60,12 +60,12 @@ namespace {
 
 // Scratch elements one axis pass of `count` lines of length n needs: lines
 // whose padded length exceeds one tile run the four-step path through
-// count * M elements (twice that for three-factor Bluestein lines).
+// count * M elements.
 uint64_t pass_scratch(uint32_t n, uint32_t count)
 {
     const int m = mdft::compute_m(n);  // "Error size <n> for compute m" outside [1, 2^29]
     if (n <= 1) return 0;
-    return mdft::large_scratch_elems(n, m, !mdft::is_pow2(n), count);
+    return mdft::large_scratch_elems(m, count);
 }
 
 }  // namespace

@@ -61,11 +61,9 @@ void large_split(int logm, int* l1, int* l2, int* l3)
     }
 }
 
-uint64_t large_scratch_elems(long long n, long long m, bool blue, long long count)
+uint64_t large_scratch_elems(long long m, long long count)
 {
-    if (m <= LineMaxTile) return 0;
-    const uint64_t per = (uint64_t)count * (uint64_t)m;
-    return blue && ilog2((uint64_t)m) >= three_factor_min_logm() ? 2 * per : per;
+    return m > LineMaxTile ? (uint64_t)count * (uint64_t)m : 0;
 }
 
 int line_shape(long long n, bool cols)
@@ -378,12 +376,10 @@ static LargeJob large_job(const float2* tw_lo, const float2* tw_hi, bool inverse
     return j;
 }
 
-void Context::enqueue_large_fft3(const LargeFft& f, const float2* src, int64_t es, int64_t ls, float2* dst,
-                                 int64_t tes, int64_t tls, int64_t count, const LargeEnds& e, float2* T,
-                                 hipStream_t stream) const
+void Context::enqueue_large_a1a2(const LargeFft& f, const float2* src, int64_t es, int64_t ls, int64_t count,
+                                 const LargeEnds& e, float2* T, hipStream_t stream) const
 {
     const int64_t M = f.m, M1 = int64_t(1) << f.logm1, M2 = int64_t(1) << f.logm2, M3 = int64_t(1) << f.logm3;
-    const int64_t Mr = M2 * M3;
     // A1: per (line, j = n1 + M1*n2) the M3-point FFT over n3 (input stride
     // M1*M2), times w_Mr^(n2*k3) = w_M^(M1*n2*k3), into T[line*M + j*M3 + k3]
     LargeJob a = large_job(f.tw_lo, f.tw_hi, e.inverse, e.n);
@@ -404,6 +400,15 @@ void Context::enqueue_large_fft3(const LargeFft& f, const float2* src, int64_t e
     b.count = count * M1 * M3;
     b.load_mode = LARGE_LOAD_PLAIN, b.store_mode = LARGE_STORE_PLAIN;
     large_step(f.logm2, twiddles(f.logm2), b, cu_count_, stream);
+}
+
+void Context::enqueue_large_fft3(const LargeFft& f, const float2* src, int64_t es, int64_t ls, float2* dst,
+                                 int64_t tes, int64_t tls, int64_t count, const LargeEnds& e, float2* T,
+                                 hipStream_t stream) const
+{
+    const int64_t M = f.m, M1 = int64_t(1) << f.logm1, M3 = int64_t(1) << f.logm3;
+    const int64_t Mr = M / M1;
+    enqueue_large_a1a2(f, src, es, ls, count, e, T, stream);
     // C: per (line, kr = k3 + M3*k2), element n1 at line*M + n1*M3 + k3 +
     // k2*M1*M3, times w_M^(n1*kr) on the load, the M1-point FFT over n1, stored
     // at kr + Mr*k1
@@ -416,6 +421,49 @@ void Context::enqueue_large_fft3(const LargeFft& f, const float2* src, int64_t e
     c.store_mode = e.store_mode, c.kmul = Mr, c.scale = e.scale;
     c.chirp = e.chirp, c.dtab = e.dtab;
     large_step(f.logm1, twiddles(f.logm1), c, cu_count_, stream);
+}
+
+void Context::enqueue_large_blue3(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
+                                  int64_t count, bool inverse, float2* T, hipStream_t stream) const
+{
+    const LargeFft f{lp.m, lp.logm1, lp.logm2, lp.logm3, lp.tw_lo, lp.tw_hi};
+    const int64_t M = f.m, M1 = int64_t(1) << f.logm1, M2 = int64_t(1) << f.logm2, M3 = int64_t(1) << f.logm3;
+    const int64_t Mr = M2 * M3;
+    // the chirped line's Mr-point FFTs (x * c, zero past n)
+    const LargeEnds e{LARGE_LOAD_CHIRP, LARGE_STORE_PLAIN, inverse, lp.n, 1.0f, lp.chirp, nullptr};
+    enqueue_large_a1a2(f, src, es, ls, count, e, T, stream);
+    // C fused, in place: per (line, kr) times w_M^(n1*kr), the M1-point FFT
+    // over n1, conj(v * D) at kr + Mr*k1, the M1-point FFT over k1 (the second
+    // transform with the factors' roles exchanged, j = j1 + M1*jr), times
+    // w_M^(kr*j1), j1 stored in the slot of n1
+    LargeJob c = large_job(f.tw_lo, f.tw_hi, inverse, lp.n);
+    c.src = T, c.dst = T;
+    c.sub_in = Mr, c.ls_in = M, c.sub2_in = M3, c.ls2_in = 1, c.ls3_in = M1 * M3, c.es_in = M3;
+    c.sub_out = Mr, c.ls_out = M, c.sub2_out = M3, c.ls2_out = 1, c.ls3_out = M1 * M3, c.es_out = M3;
+    c.count = count * Mr;
+    c.load_mode = LARGE_LOAD_TWIDDLE, c.store_mode = LARGE_STORE_DIAG_FFT, c.tmod = Mr, c.kmul = Mr;
+    c.dtab = lp.dtab;
+    large_step(f.logm1, twiddles(f.logm1), c, cu_count_, stream);
+    // X1, in place: per (line, j1*M3 + k3) the M2-point FFT over k2, times w_Mr^(k3*jr2) = w_M^(M1*k3*jr2)
+    LargeJob x1 = large_job(f.tw_lo, f.tw_hi, inverse, lp.n);
+    x1.src = T, x1.dst = T;
+    x1.sub_in = M1 * M3, x1.ls_in = M, x1.ls2_in = 1, x1.es_in = M1 * M3;
+    x1.sub_out = M1 * M3, x1.ls_out = M, x1.ls2_out = 1, x1.es_out = M1 * M3;
+    x1.count = count * M1 * M3;
+    x1.load_mode = LARGE_LOAD_PLAIN, x1.store_mode = LARGE_STORE_TWIDDLE;
+    x1.tmod = M3, x1.tdiv = 1, x1.tmul = M1;
+    large_step(f.logm2, twiddles(f.logm2), x1, cu_count_, stream);
+    // X2: per (line, r = j1 + M1*jr2), element k3 at line*M + j1*M3 + jr2*M1*M3 + k3,
+    // the M3-point FFT over k3 into j = r + M1*M2*jr3: conj(v) * c, crop to n, 1/n for the inverse
+    LargeJob x2 = large_job(f.tw_lo, f.tw_hi, inverse, lp.n);
+    x2.src = T, x2.dst = dst;
+    x2.sub_in = M1 * M2, x2.ls_in = M, x2.sub2_in = M1, x2.ls2_in = M3, x2.ls3_in = M1 * M3, x2.es_in = 1;
+    x2.sub_out = M1 * M2, x2.ls_out = ls, x2.ls2_out = es, x2.es_out = M1 * M2 * es;
+    x2.count = count * M1 * M2;
+    x2.load_mode = LARGE_LOAD_PLAIN, x2.store_mode = LARGE_STORE_CHIRP, x2.kmul = M1 * M2;
+    x2.scale = inverse ? 1.0f / (float)lp.n : 1.0f;
+    x2.chirp = lp.chirp;
+    large_step(f.logm3, twiddles(f.logm3), x2, cu_count_, stream);
 }
 
 void Context::enqueue_large_fft(const LargeFft& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
@@ -463,13 +511,7 @@ void Context::enqueue_large_axis(const LinePlan& lp, const float2* src, float2* 
             enqueue_large_fft3(f, src, es, ls, dst, es, ls, count, e, scratch, stream);
             return;
         }
-        // the chirped line's FFT times D into U (natural order, conjugated),
-        // then the FFT of U with the chirp, crop and 1/n on the store
-        float2* U = scratch + count * M;
-        const LargeEnds e1{LARGE_LOAD_CHIRP, LARGE_STORE_DIAG, inverse, lp.n, 1.0f, lp.chirp, lp.dtab};
-        enqueue_large_fft3(f, src, es, ls, U, 1, M, count, e1, scratch, stream);
-        const LargeEnds e2{LARGE_LOAD_PLAIN, LARGE_STORE_CHIRP, inverse, lp.n, scale, lp.chirp, nullptr};
-        enqueue_large_fft3(f, U, 1, M, dst, es, ls, count, e2, scratch, stream);
+        enqueue_large_blue3(lp, src, dst, es, ls, count, inverse, scratch, stream);
         return;
     }
     float2* S = scratch;  // step A output, count * M (later steps in place or out of it)

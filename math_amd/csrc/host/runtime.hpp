@@ -53,10 +53,9 @@ inline int shape_index(int shape) { return is_f3(shape) ? MAX_LOGM + 1 + (f3_k(s
 int three_factor_min_logm();
 // the factors (log2) of a padded length 2^logm above one tile; l3 = 0: two
 void large_split(int logm, int* l1, int* l2, int* l3);
-// scratch of one axis pass over `count` lines of length n, padded length m:
-// count * m elements above one tile, twice that for a three-factor Bluestein
-// line (its product between the two FFTs), none otherwise
-uint64_t large_scratch_elems(long long n, long long m, bool blue, long long count);
+// scratch of one axis pass over `count` lines of padded length m: count * m
+// elements above one tile, none otherwise
+uint64_t large_scratch_elems(long long m, long long count);
 
 // Device tables of one line length n.
 struct LinePlan {
@@ -79,7 +78,7 @@ struct LinePlan {
     uint64_t bytes = 0;
     bool large() const { return logm > MAX_LOGM; }
     // scratch (complex<float> elements) one axis pass over `count` lines needs
-    uint64_t scratch_elems(int count) const { return large_scratch_elems(n, m, blue, count); }
+    uint64_t scratch_elems(int count) const { return large_scratch_elems(m, count); }
     ~LinePlan();
 };
 
@@ -95,7 +94,7 @@ struct LargeFft {
 // The first step's load and the last step's store of a three-factor FFT.
 struct LargeEnds {
     int load_mode = 0;   // LargeLoad (plain, conj, chirp)
-    int store_mode = 0;  // LargeStore (conj with scale, diag, chirp)
+    int store_mode = 0;  // LargeStore of the last step (conj with scale)
     bool inverse = false;
     int64_t n = 0;       // line length (the chirp load's zero padding, the chirp store's crop)
     float scale = 1.0f;
@@ -143,6 +142,12 @@ public:
     // through T (count * m elements); e gives the ends' load and store.
     void enqueue_large_fft3(const LargeFft& f, const float2* src, int64_t es, int64_t ls, float2* dst, int64_t tes,
                             int64_t tls, int64_t count, const LargeEnds& e, float2* T, hipStream_t stream) const;
+    // its first two steps (A1 with e's load, A2), into T
+    void enqueue_large_a1a2(const LargeFft& f, const float2* src, int64_t es, int64_t ls, int64_t count,
+                            const LargeEnds& e, float2* T, hipStream_t stream) const;
+    // a three-factor Bluestein line in five steps through T (count * m elements)
+    void enqueue_large_blue3(const LinePlan& lp, const float2* src, float2* dst, int64_t es, int64_t ls,
+                             int64_t count, bool inverse, float2* T, hipStream_t stream) const;
     // Forward power-of-two four-step FFT of length f.m, times scale
     // (scratch: count * m elements).
     void enqueue_large_fft(const LargeFft& f, const float2* src, float2* dst, int64_t es, int64_t ls, int64_t count,

@@ -25,8 +25,12 @@
 //       FFT of every n1, its output kr spread as k3 + M1*M3*k2 beside n1);
 //   C:  per (line, kr) times w_M^(n1*kr) on the load, the M1-point FFT over n1,
 //       stored at kr + Mr*k1.
-// A Bluestein line runs A1 (chirp), A2, C (times D), then A1, A2, C (chirp,
-// crop) on the product, through two scratch buffers.
+// A Bluestein line runs A1 (chirp) and A2, then C in place fused like B: the
+// M1-point FFT, conj(v * D), the M1-point FFT over k1 and w_M^(kr*j1), with j1
+// in the slot of n1; the second transform's Mr-point FFT over kr then runs as
+//   X1: in place, per (line, j1, k3) the M2-point FFT over k2, times w_Mr^(k3*jr2);
+//   X2: per (line, j1, jr2) the M3-point FFT over k3 (contiguous), stored at
+//       j = j1 + M1*(jr2 + M2*jr3) with the chirp and crop.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -35,19 +39,14 @@
 
 namespace mdft {
 
-// line L of a step maps to (L / sub) * ls + (L % sub) * ls2; element p to p * es
-__device__ __forceinline__ int64_t line_base(int64_t L, int64_t sub, int64_t ls, int64_t ls2)
+// line L of a step maps to (L / sub) * ls + r * ls2 with r = L % sub, element
+// p to p * es; r may split once more (sub2 > 0): (r % sub2) * ls2 + (r / sub2) * ls3
+__device__ __forceinline__ int64_t line_base3(int64_t L, int64_t sub, int64_t ls, int64_t ls2, int64_t sub2,
+                                              int64_t ls3)
 {
-    return (L / sub) * ls + (L % sub) * ls2;
-}
-
-// the input side, whose r = L % sub may split once more (sub2 > 0)
-__device__ __forceinline__ int64_t line_base_in(int64_t L, const LargeJob& job)
-{
-    const int64_t r = L % job.sub_in;
-    const int64_t off = job.sub2_in > 0 ? (r % job.sub2_in) * job.ls2_in + (r / job.sub2_in) * job.ls3_in
-                                        : r * job.ls2_in;
-    return (L / job.sub_in) * job.ls_in + off;
+    const int64_t r = L % sub;
+    const int64_t off = sub2 > 0 ? (r % sub2) * ls2 + (r / sub2) * ls3 : r * ls2;
+    return (L / sub) * ls + off;
 }
 
 // w_M^e, e < M, from the two-level table
@@ -75,7 +74,8 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
 
     float2 v[C::EPT];
     {
-        const float2* s = job.src + (valid ? line_base_in(line, job) : 0);
+        const float2* s =
+            job.src + (valid ? line_base3(line, job.sub_in, job.ls_in, job.ls2_in, job.sub2_in, job.ls3_in) : 0);
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
             const int64_t pos = t + i * C::TPL;
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
     }
 
     if (!valid) return;
-    float2* o = job.dst + line_base(line, job.sub_out, job.ls_out, job.ls2_out);
+    float2* o = job.dst + line_base3(line, job.sub_out, job.ls_out, job.ls2_out, job.sub2_out, job.ls3_out);
 #pragma unroll
     for (int i = 0; i < C::EPT; ++i) {
         const int64_t pos = t + i * C::TPL;
@@ -140,7 +140,8 @@ static hipError_t launch_large_tt(const LargeJob& job, hipStream_t stream)
     constexpr int LPW = LineCfg<LOGM, T>::LPW;
     const int64_t grid = (job.count + LPW - 1) / LPW;
     if (grid <= 0) return hipSuccess;
-    if (grid > 0x7fffffff || job.sub_in < 1 || job.sub_out < 1 || job.sub2_in < 0) return hipErrorInvalidValue;
+    if (grid > 0x7fffffff || job.sub_in < 1 || job.sub_out < 1 || job.sub2_in < 0 || job.sub2_out < 0)
+        return hipErrorInvalidValue;
     const bool twiddled = job.load_mode == LARGE_LOAD_TWIDDLE || job.store_mode == LARGE_STORE_TWIDDLE ||
                           job.store_mode == LARGE_STORE_DIAG_FFT;
     if (twiddled && (job.tmod < 1 || job.tdiv < 1 || job.tmul < 1)) return hipErrorInvalidValue;

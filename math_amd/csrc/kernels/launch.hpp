@@ -52,6 +52,7 @@ struct LargeJob {
     int64_t ls_in, ls2_in, es_in, sub_in;
     int64_t sub2_in, ls3_in;
     int64_t ls_out, ls2_out, es_out, sub_out;
+    int64_t sub2_out, ls3_out;
     int64_t tmod, tdiv, tmul;
     int64_t count;
     int load_mode, store_mode, inverse, pad;

@@ -348,8 +348,10 @@ def single_transforms(M, torch, reps=20):
     single-image configurations 640x480 (forward and inverse), 1024x1024 (round
     trip) and 4093x2047, with their algorithmic bandwidth 32*w*h bytes per
     transform; lines longer than one tile (20000x3, 3x32768, 65536x1,
-    16385x64: the four-step path); also the image path on a 640x480 rectangle
-    and 4096 FFTs of 4096 points (16 bytes per point)."""
+    16385x64: the four-step path) and the longest sides the reference accepts
+    (2^29 and 2^29 - 1, padded 2^29 and 2^30: three factors; their error is the
+    device round trip's, forward then inverse); also the image path on a
+    640x480 rectangle and 4096 FFTs of 4096 points (16 bytes per point)."""
     res = {}
     sh = torch.cuda.current_stream().cuda_stream
     g = torch.Generator(device="cuda:0")
@@ -406,6 +408,20 @@ def single_transforms(M, torch, reps=20):
                                "bound": float("%.3e" % (2e-6 * max(1.0, math.log2(_padded(w) * _padded(h)))))}
         p.close()
         del x, y
+    for (w, h) in [(1 << 29, 1), ((1 << 29) - 1, 1)]:
+        x = torch.randn(w * h, dtype=torch.complex64, device="cuda:0", generator=g)
+        y = torch.empty_like(x)
+        p = M.Plan(w, h)
+        us = _event_us(torch, lambda: p.exec_device(False, x.data_ptr(), y.data_ptr(), sh), 3)
+        p.exec_device(True, y.data_ptr(), y.data_ptr(), sh)
+        torch.cuda.synchronize()
+        rt = (torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x)).item()
+        res[f"{w}x{h}_fwd"] = {"us": round(us, 1), "alg_gbs": round(32.0 * w * h / (us * 1e-6) / 1e9, 1),
+                               "round_trip_nl2": float("%.3e" % rt),
+                               "bound": float("%.3e" % (2e-6 * math.log2(_padded(w) * _padded(h))))}
+        p.close()
+        del x, y
+        torch.cuda.empty_cache()
     f = M.Fft(4096, 4096)
     z = torch.randn(4096 * 4096, dtype=torch.complex64, device="cuda:0", generator=g)
     us = _event_us(torch, lambda: f.exec_device(False, z.data_ptr(), z.data_ptr(), sh), reps)

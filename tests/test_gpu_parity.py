@@ -130,8 +130,9 @@ def test_all_line_lengths_1_to_300(mdft):
 
 # 1025-1536 and 2049-3072 (and 2500) run 3 * 2^k convolutions along both axes, 3073 the next power of two;
 # 384, 768, 1536, 3072 and 6144 are 3 * 2^k themselves (direct FFTs)
+# 8193 is the longest Bluestein line one tile holds (16384-point convolution), 16383 the shortest above it
 @pytest.mark.parametrize("n", [384, 511, 512, 513, 768, 1000, 1023, 1024, 1025, 1536, 2047, 2048, 2049, 2500, 3072,
-                               3073, 4093, 4096, 4097, 6144, 8191, 8192, 8193])
+                               3073, 4093, 4096, 4097, 6144, 8191, 8192, 8193, 8194, 16383, 16384])
 def test_long_lines(mdft, n):
     rng = np.random.default_rng(n)
     x = rand_c64(rng, (2, n))

@@ -39,14 +39,16 @@
 
 namespace mdft {
 
+__device__ __forceinline__ int64_t low_bits(int64_t v, int lg) { return v & ((int64_t(1) << lg) - 1); }
+
 // line L of a step maps to (L / sub) * ls + r * ls2 with r = L % sub, element
-// p to p * es; r may split once more (sub2 > 0): (r % sub2) * ls2 + (r / sub2) * ls3
-__device__ __forceinline__ int64_t line_base3(int64_t L, int64_t sub, int64_t ls, int64_t ls2, int64_t sub2,
-                                              int64_t ls3)
+// p to p * es; r may split once more (sub2 > 0): (r % sub2) * ls2 + (r / sub2) * ls3.
+// sub and sub2 are powers of two (lg: their log2, lg2 = -1 without the split).
+__device__ __forceinline__ int64_t line_base3(int64_t L, int lg, int64_t ls, int64_t ls2, int lg2, int64_t ls3)
 {
-    const int64_t r = L % sub;
-    const int64_t off = sub2 > 0 ? (r % sub2) * ls2 + (r / sub2) * ls3 : r * ls2;
-    return (L / sub) * ls + off;
+    const int64_t r = low_bits(L, lg);
+    const int64_t off = lg2 >= 0 ? low_bits(r, lg2) * ls2 + (r >> lg2) * ls3 : r * ls2;
+    return (L >> lg) * ls + off;
 }
 
 // w_M^e, e < M, from the two-level table
@@ -68,14 +70,14 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
     const bool valid = line < job.count;
     const uint32_t sgn = job.inverse ? 0x80000000u : 0u;
     // sub-line coordinate inside the long line: n1 (step A) or k2 (step C)
-    const int64_t sub_idx = line % job.sub_in;
+    const int64_t sub_idx = low_bits(line, job.lg_sub_in);
     // twiddle exponent per element of this sub-line
-    const int64_t tq = job.tmod > 0 && job.tdiv > 0 ? ((line % job.tmod) / job.tdiv) * job.tmul : 0;
+    const int64_t tq = (low_bits(line, job.lg_tmod) >> job.lg_tdiv) * job.tmul;
 
     float2 v[C::EPT];
     {
         const float2* s =
-            job.src + (valid ? line_base3(line, job.sub_in, job.ls_in, job.ls2_in, job.sub2_in, job.ls3_in) : 0);
+            job.src + (valid ? line_base3(line, job.lg_sub_in, job.ls_in, job.ls2_in, job.lg_sub2_in, job.ls3_in) : 0);
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
             const int64_t pos = t + i * C::TPL;
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
     }
 
     FftTile<LOGM, T, true>::run(v, t, ll, lds, job.tw);
-    const int64_t q_line = line % job.sub_out;
+    const int64_t q_line = low_bits(line, job.lg_sub_out);
     if constexpr (FUSED) {
         // conj(v * D) at k = k2 + M2*k1 (q_line = k2 < M2, pos = k1 < M1: in bounds on every line),
         // then the next length-M1 FFT over k1; its outputs j1 take the twiddle below
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
     }
 
     if (!valid) return;
-    float2* o = job.dst + line_base3(line, job.sub_out, job.ls_out, job.ls2_out, job.sub2_out, job.ls3_out);
+    float2* o = job.dst + line_base3(line, job.lg_sub_out, job.ls_out, job.ls2_out, job.lg_sub2_out, job.ls3_out);
 #pragma unroll
     for (int i = 0; i < C::EPT; ++i) {
         const int64_t pos = t + i * C::TPL;
@@ -134,17 +136,40 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
     }
 }
 
+// log2 of a power of two, -1 for anything else
+static int lg_of(int64_t v)
+{
+    if (v < 1 || (v & (v - 1)) != 0) return -1;
+    int r = 0;
+    while ((int64_t(1) << r) < v) ++r;
+    return r;
+}
+
 template <int LOGM, int T>
-static hipError_t launch_large_tt(const LargeJob& job, hipStream_t stream)
+static hipError_t launch_large_tt(const LargeJob& job0, hipStream_t stream)
 {
     constexpr int LPW = LineCfg<LOGM, T>::LPW;
-    const int64_t grid = (job.count + LPW - 1) / LPW;
+    const int64_t grid = (job0.count + LPW - 1) / LPW;
     if (grid <= 0) return hipSuccess;
-    if (grid > 0x7fffffff || job.sub_in < 1 || job.sub_out < 1 || job.sub2_in < 0 || job.sub2_out < 0)
+    if (grid > 0x7fffffff) return hipErrorInvalidValue;
+    // the kernel splits sub-line indices with shifts and masks
+    LargeJob job = job0;
+    job.lg_sub_in = lg_of(job.sub_in);
+    job.lg_sub_out = lg_of(job.sub_out);
+    job.lg_sub2_in = job.sub2_in > 0 ? lg_of(job.sub2_in) : -1;
+    job.lg_sub2_out = job.sub2_out > 0 ? lg_of(job.sub2_out) : -1;
+    if (job.lg_sub_in < 0 || job.lg_sub_out < 0 || job.sub2_in < 0 || job.sub2_out < 0 ||
+        (job.sub2_in > 0 && job.lg_sub2_in < 0) || (job.sub2_out > 0 && job.lg_sub2_out < 0))
         return hipErrorInvalidValue;
     const bool twiddled = job.load_mode == LARGE_LOAD_TWIDDLE || job.store_mode == LARGE_STORE_TWIDDLE ||
                           job.store_mode == LARGE_STORE_DIAG_FFT;
-    if (twiddled && (job.tmod < 1 || job.tdiv < 1 || job.tmul < 1)) return hipErrorInvalidValue;
+    if (twiddled) {
+        job.lg_tmod = lg_of(job.tmod);
+        job.lg_tdiv = lg_of(job.tdiv);
+        if (job.lg_tmod < 0 || job.lg_tdiv < 0 || job.tmul < 1) return hipErrorInvalidValue;
+    } else {
+        job.lg_tmod = 0, job.lg_tdiv = 0;  // exponent 0
+    }
     if (job.store_mode == LARGE_STORE_DIAG_FFT)
         hipLaunchKernelGGL((k_large_step<LOGM, T, true>), dim3((unsigned)grid), dim3(T), 0, stream, job);
     else

@@ -46,6 +46,7 @@ enum LargeStore { LARGE_STORE_PLAIN = 0, LARGE_STORE_TWIDDLE = 1, LARGE_STORE_DI
 // ls3_in in place of r * ls2_in), and writes likewise with the *_out fields.
 // The twiddle of LARGE_LOAD_TWIDDLE, LARGE_STORE_TWIDDLE and
 // LARGE_STORE_DIAG_FFT is w_M^e of element pos, e = ((L % tmod) / tdiv) * tmul * pos.
+// sub_*, sub2_*, tmod and tdiv are powers of two.
 struct LargeJob {
     const float2* src;
     float2* dst;
@@ -58,12 +59,15 @@ struct LargeJob {
     int load_mode, store_mode, inverse, pad;
     int64_t n, m1, m2;
     int64_t kmul;  // natural index of the store modes DIAG, DIAG_FFT, CHIRP: q + kmul * pos
+    // log2 of sub_in, sub2_in, sub_out, sub2_out, tmod, tdiv (all powers of two; -1: no second
+    // split), filled in by launch_large_step
     float scale;
     const float2* chirp;
     const float2* dtab;
     const float2* tw_lo;
     const float2* tw_hi;
     const float2* tw;  // Stockham rows of the sub-FFT length
+    int lg_sub_in, lg_sub2_in, lg_sub_out, lg_sub2_out, lg_tmod, lg_tdiv;
 };
 
 // min_blocks: smaller tiles when 1024-thread ones would give fewer workgroups

@@ -281,9 +281,11 @@ int mdft_batch_destroy(mdft_batch batch);
  * and is admitted in arrival order when its slot, the slots in use and the
  * resident line-plan tables fit the budget.  A scheduler thread
  * launches the admitted requests as micro-batches on `streams` streams (at
- * most `max_batch` requests each), so requests arriving while others run
- * join the next launch and overlap with them.  submit/wait/release/stats are
- * thread-safe; destroy must not race with them. */
+ * most `max_batch` requests each; a request of 2^22 points or more runs in a
+ * micro-batch of its own, so smaller ones do not complete with it), so
+ * requests arriving while others run join the next launch and overlap with
+ * them.  submit/wait/release/stats are thread-safe; destroy must not race
+ * with them. */
 
 typedef struct mdft_server_s* mdft_server;
 

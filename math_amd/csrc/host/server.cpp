@@ -11,7 +11,10 @@
 //   batch job tables and work lists of batch_builder.cpp, one workgroup per
 //   tile) and launches each on the next free stream, so requests that arrive
 //   while others run join the next launch and overlap with them on the
-//   device.  Completion is observed per micro-batch with an event.
+//   device.  Completion is observed per micro-batch with an event, so a
+//   request of kIsolatePoints or more runs in a micro-batch of its own: the
+//   small requests behind it in the queue launch on the other streams instead
+//   of completing with it.
 // * The row pass reads the client's device input directly and writes the
 //   slot; the column pass works in the slot.  The result stays in the slot
 //   until the client releases it.  Line plans (chirp and diagonal tables) are
@@ -34,6 +37,10 @@ using mdft::Context;
 using mdft::Error;
 
 namespace {
+
+// requests of this many points or more launch alone (4 Mi points, 32 MiB of
+// complex<float>: about 4 % of a stream with sides log-uniform in [32, 4096])
+constexpr uint64_t kIsolatePoints = uint64_t(1) << 22;
 
 using Clock = std::chrono::steady_clock;
 
@@ -288,6 +295,8 @@ void scheduler(mdft_server_s* s)
                     holding = true;
                 while (!holding && !s->pending.empty() && entries.size() < s->max_batch) {
                     Request& r = s->reqs[s->pending.front()];
+                    const bool alone = (uint64_t)r.w * r.h >= kIsolatePoints;
+                    if (alone && !entries.empty()) break;  // the next free stream takes it
                     uint64_t off = 0;
                     // slots in use + this slot + resident tables within the budget
                     if (s->arena.used() + slot_bytes(r.bytes) + s->table_bytes > s->arena.size()) break;
@@ -303,6 +312,7 @@ void scheduler(mdft_server_s* s)
                     }
                     free_lane->reqs.push_back(s->pending.front());
                     s->pending.pop_front();
+                    if (alone) break;
                 }
             }
             if (!entries.empty()) {

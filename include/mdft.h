@@ -309,7 +309,9 @@ typedef struct {
     double arrival_us; /* arrival time from the start of the replay */
 } mdft_request;
 
-/* streams = 0 and max_batch = 0 select the defaults (4 and 128). */
+/* streams = 0 and max_batch = 0 select the defaults (4 and 128).  Create runs
+ * one tiny micro-batch on the arena (budgets of 18 KiB and more) so the batch
+ * kernels are loaded before the first request. */
 int mdft_server_create(mdft_server* server, uint64_t hbm_budget_bytes, uint32_t streams, uint32_t max_batch);
 
 /* Batching window (no counterpart in the reference): while another

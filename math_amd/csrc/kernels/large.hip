@@ -69,7 +69,7 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
     const int64_t line = int64_t(xcd_remap(blockIdx.x, gridDim.x)) * C::LPW + ll;
     const bool valid = line < job.count;
     const uint32_t sgn = job.inverse ? 0x80000000u : 0u;
-    // sub-line coordinate inside the long line: n1 (step A) or k2 (step C)
+    // sub-line coordinate inside the long line (the chirp load's element offset)
     const int64_t sub_idx = low_bits(line, job.lg_sub_in);
     // twiddle exponent per element of this sub-line
     const int64_t tq = (low_bits(line, job.lg_tmod) >> job.lg_tdiv) * job.tmul;
@@ -102,8 +102,8 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
     FftTile<LOGM, T, true>::run(v, t, ll, lds, job.tw);
     const int64_t q_line = low_bits(line, job.lg_sub_out);
     if constexpr (FUSED) {
-        // conj(v * D) at k = k2 + M2*k1 (q_line = k2 < M2, pos = k1 < M1: in bounds on every line),
-        // then the next length-M1 FFT over k1; its outputs j1 take the twiddle below
+        // conj(v * D) at k = k2 + M2*k1 (q_line = k2 < M2, pos = k1 < M1; three factors: kr + Mr*k1;
+        // in bounds on every line), then the next length-M1 FFT over k1; its outputs j1 take the twiddle below
 #pragma unroll
         for (int i = 0; i < C::EPT; ++i) {
             const int64_t k = q_line + job.kmul * (t + i * C::TPL);
@@ -119,7 +119,8 @@ __global__ __launch_bounds__(T) void k_large_step(LargeJob job)
         const int64_t pos = t + i * C::TPL;
         float2 y = v[i];
         if (job.store_mode == LARGE_STORE_TWIDDLE || job.store_mode == LARGE_STORE_DIAG_FFT) {
-            // times w_M^(n1*k2) (two factors: n1*k2 < M1*M2 = M), or w_Mr^(n2*k3) = w_M^(M1*n2*k3) (A1 of three)
+            // the step's twiddle w_M^(tq*pos), tq*pos < M: n1*k2 (A), k2*j1 (B), M1*n2*k3 (A1),
+            // kr*j1 (fused C), M1*k3*jr2 (X1)
             y = cmul(y, large_twiddle(job, tq * pos));
         } else if (job.store_mode == LARGE_STORE_DIAG) {
             // natural index k = k2 + M2*k1; forward conj(v*D), inverse conj(v*conj(D))

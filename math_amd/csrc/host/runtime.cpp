@@ -174,6 +174,13 @@ Context::Context(int device) : device_(device)
     // FP64 table scratch for every tile-resident Bluestein length (2n + m doubles)
     scratch64_doubles_ = 2 * (size_t(LineMaxTile) / 2 + 1) + size_t(LineMaxTile);
     hip_check(hipMalloc(&scratch64_, sizeof(double) * scratch64_doubles_), "hipMalloc scratch");
+    // load every kernel file's code now, not in a caller's first transform
+    uint32_t* word = reinterpret_cast<uint32_t*>(scratch64_);
+    hip_check(warm_axis(word, gen_stream_), "kernel load");
+    hip_check(warm_image(word, gen_stream_), "kernel load");
+    hip_check(warm_large(word, gen_stream_), "kernel load");
+    hip_check(warm_batch(word, gen_stream_), "kernel load");
+    hip_check(hipStreamSynchronize(gen_stream_), "kernel load");
     noalloc_setup(device);
     // default tile shapes: wide tiles for columns of 1024 points and more
     for (int i = 0; i < NSHAPES; ++i) {

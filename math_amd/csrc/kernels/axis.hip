@@ -74,6 +74,18 @@ static hipError_t launch_axis_t(const AxisJob& job, bool blue, bool cols, int th
     return launch_axis_tt<S, TN>(job, blue, cols, stream);
 }
 
+// one word written, so the context loads this file's kernels at creation
+__global__ void k_warm_axis(uint32_t* word)
+{
+    if (threadIdx.x == 0) word[0] = 0u;
+}
+
+hipError_t warm_axis(uint32_t* word, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_warm_axis, dim3(1), dim3(64), 0, stream, word);
+    return hipGetLastError();
+}
+
 hipError_t launch_axis(int shape, const AxisJob& job, bool blue, bool cols, int threads, hipStream_t stream)
 {
     switch (shape) {

@@ -239,6 +239,12 @@ hipError_t launch_batch_grid(int threads, const BatchJob* jobs, const uint2* ite
     return hipGetLastError();
 }
 
+hipError_t warm_batch(uint32_t* word, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, word, 1u, 0u);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_persistent(int threads, const BatchJob* jobs, const uint2* items, uint32_t nitems,
                                    uint32_t* ctl, uint32_t nctl, int grid, uint32_t spin_limit, hipStream_t stream)
 {

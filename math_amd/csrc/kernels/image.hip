@@ -124,6 +124,18 @@ static unsigned pixel_grid(int64_t npix)
     return (unsigned)(g < 65536 ? g : 65536);
 }
 
+// one word written, so the context loads this file's kernels at creation
+__global__ void k_warm_image(uint32_t* word)
+{
+    if (threadIdx.x == 0) word[0] = 0u;
+}
+
+hipError_t warm_image(uint32_t* word, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_warm_image, dim3(1), dim3(64), 0, stream, word);
+    return hipGetLastError();
+}
+
 hipError_t launch_image_luminance(const ImageIO& io, float2* dst, int w, int h, hipStream_t stream)
 {
     const int64_t npix = int64_t(w) * h;

@@ -198,6 +198,18 @@ static hipError_t launch_large_t(const LargeJob& job, int min_blocks, hipStream_
     return launch_large_tt<LOGM, 1024>(job, stream);
 }
 
+// one word written, so the context loads this file's kernels at creation
+__global__ void k_warm_large(uint32_t* word)
+{
+    if (threadIdx.x == 0) word[0] = 0u;
+}
+
+hipError_t warm_large(uint32_t* word, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_warm_large, dim3(1), dim3(64), 0, stream, word);
+    return hipGetLastError();
+}
+
 hipError_t launch_large_step(int logm, const LargeJob& job, int min_blocks, hipStream_t stream)
 {
     switch (logm) {

@@ -75,6 +75,15 @@ hipError_t launch_large_step(int logm, const LargeJob& job, int min_blocks, hipS
 hipError_t gen_large_twiddles(float2* lo, float2* hi, int64_t m, hipStream_t stream);
 hipError_t gen_h2_f32(float2* h2, const double2* chirp64, int64_t n, int64_t m, hipStream_t stream);
 
+// A one-word launch from each kernel file (axis, image, long lines, batch): the
+// first launch from a file loads all of its kernels (7.7 ms measured for the
+// axis kernels), which the context does at creation rather than in a caller's
+// first transform.
+hipError_t warm_axis(uint32_t* word, hipStream_t stream);
+hipError_t warm_image(uint32_t* word, hipStream_t stream);
+hipError_t warm_large(uint32_t* word, hipStream_t stream);
+hipError_t warm_batch(uint32_t* word, hipStream_t stream);
+
 hipError_t gen_twiddles(float2* table, hipStream_t stream);
 // Direction tables of a tile-resident Bluestein length (float2 entries):
 // [load fwd c | load inv conj(c) | D | conj(D) | store fwd conj(c) | store inv c/n | m zeros]
